@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Semblance throughput on one MI355X: grid ("linear") search over a whole CMP line.
+
+One step = one pass of the hot path over the line: every (midpoint, t0, velocity)
+semblance of the 500-midpoint line (fold 48-96, 1501 samples at 2 ms, 1024 velocities
+1400-5000 m/s, 11-sample window) with the best-velocity pick fused in, in one launch.
+Inputs are resident in HBM before the timed region starts.  Prints ONE JSON line.
+
+Roofline: the algorithmic bytes of a launch are the gathered-sample stream,
+sum over (midpoint, t0, candidate) of fold * (w+1) * sample_bytes, divided by the
+launch's average duration (HIP events on the launch stream).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU work budget of the oracle baseline sample (0 disables it)")
+    return ap.parse_args()
+
+
+def read_traffic(workload, storage):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel calibrated), when a summary for this workload is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_grid_{workload}_{storage}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(line, tau, params, begins, counts, m0s, seconds):
+    """The CPU oracle (C restatement, OpenMP over t0) on the first midpoints of the same
+    line until `seconds` of work are done."""
+    import numpy as np
+
+    import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.lib()
+    evals = 0
+    t_start = time.perf_counter()
+    done = 0
+    for g in range(len(begins)):
+        sl = slice(int(begins[g]), int(begins[g] + counts[g]))
+        og = O.Gather(line.samples[sl], line.mid[sl], line.hoff[sl], line.dt, tau,
+                      m0=float(m0s[g]))
+        O.grid_f32(O.CMP, og, params, nthreads=threads, full=False)
+        evals += line.ns * (params.size)
+        done += 1
+        if time.perf_counter() - t_start >= seconds:
+            break
+    el = time.perf_counter() - t_start
+    return {"value": evals / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(begins)} midpoints of the same line "
+                      f"({evals} evals, {el:.1f} s), FP32 C oracle, OpenMP over t0"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(midpoints shard across GPUs without communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from highperformanceseismicstacking_amd import CMP, EXACT, FAST, F32, F16, BF16, capi, synth
+
+    storage = {"f32": F32, "f16": F16, "bf16": BF16}[args.storage]
+    variant = {"fast": FAST, "exact": EXACT}[args.variant]
+    line, cfg = synth.workload(args.workload)
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    w = 2 * tau + 1
+    lo, hi, G = cfg["lower"][0], cfg["upper"][0], cfg["granularity"][0]
+    step = float(np.float32(np.float32(hi - lo) / np.float32(G)))
+    begins, counts, m0s = line.cmp_ranges()
+
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    mp = capi.DeviceMidpoints(begins, counts, m0s)
+    params = capi.build_parameter_array(CMP, [lo], [step], [G])
+    result = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def one_step():
+        capi.grid_semblance(dl, mp, CMP, params, G, variant=variant, result=result)
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        one_step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+
+    n_evals = int(mp.n) * line.ns * G
+    sample_bytes = dl.sample_bytes()
+    alg_bytes = float(np.sum(counts.astype(np.float64))) * line.ns * G * (w + 1) * sample_bytes
+    ms_per_step = wall * 1000.0 / args.steps
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.workload, args.storage)
+
+    out = {
+        "metric": "semblance evals/s (t0 x param), CMP grid search over a line, 1 GPU",
+        "value": n_evals / (ms_per_step * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: CMP grid search, {mp.n} midpoints, fold "
+                        f"{int(counts.min())}-{int(counts.max())} (mean {counts.mean():.1f}), "
+                        f"ns={line.ns} dt={float(line.dt)*1e3:g} ms, {G} velocities "
+                        f"{lo:g}-{hi:g} m/s, w={w}",
+            "storage": args.storage,
+            "variant": args.variant,
+            "evals_per_step": n_evals,
+            "launch_ms": launch_ms,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None,
+    }
+    if args.cpu_seconds > 0:
+        pnp = params.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(line, tau, pnp, begins, counts, m0s,
+                                           args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

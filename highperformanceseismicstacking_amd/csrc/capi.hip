@@ -1,0 +1,152 @@
+// extern "C" entry points of libhss.so (declared in include/hss.h).
+#include <hip/hip_runtime.h>
+
+#include "../../include/hss.h"
+#include "kernels/grid_impl.hpp"
+#include "kernels/launch_util.hpp"
+
+namespace hss {
+
+// buildParameterArrayFor* (cmp/linear_search.cu:4-17, zocrs/linear_search.cu:4-34,
+// oct/linear_search.cu:4-28): v-major decode, storage c = 4/v^2.
+struct ParamGridArgs {
+    float lo[3];
+    float step[3];
+    unsigned cnt[3];
+    unsigned total;
+};
+
+template <int MODEL>
+__global__ void build_parameter_array_kernel(ParamGridArgs a, float *out) {
+    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.total) return;
+    if constexpr (MODEL == HSS_CMP) {
+        float v = a.lo[0] + (float)k * a.step[0];
+        out[k] = 4.0f / (v * v);
+    } else if constexpr (MODEL == HSS_ZOCRS) {
+        const unsigned iv = (k / (a.cnt[1] * a.cnt[2])) % a.cnt[0];
+        const unsigned ia = (k / a.cnt[2]) % a.cnt[1];
+        const unsigned ib = k % a.cnt[2];
+        float v = a.lo[0] + (float)iv * a.step[0];
+        out[3 * k] = 4.0f / (v * v);
+        out[3 * k + 1] = a.lo[1] + (float)ia * a.step[1];
+        out[3 * k + 2] = a.lo[2] + (float)ib * a.step[2];
+    } else {
+        const unsigned iv = (k / a.cnt[1]) % a.cnt[0];
+        const unsigned is = k % a.cnt[1];
+        float v = a.lo[0] + (float)iv * a.step[0];
+        out[2 * k] = 4.0f / (v * v);
+        out[2 * k + 1] = a.lo[1] + (float)is * a.step[1];
+    }
+}
+
+}  // namespace hss
+
+using namespace hss;
+
+extern "C" {
+
+int hss_abi_version(void) { return HSS_ABI_VERSION; }
+
+const char *hss_status_string(int status) {
+    switch (status) {
+        case HSS_OK: return "success";
+        case HSS_ERR_INVALID_MODEL: return "invalid traveltime model";
+        case HSS_ERR_INVALID_WINDOW: return "invalid window: w must be 2*tau+1 and at most 32";
+        case HSS_ERR_INVALID_SIZE: return "invalid size";
+        case HSS_ERR_NULL_POINTER: return "required pointer is NULL";
+        case HSS_ERR_INVALID_STORAGE: return "invalid sample storage type";
+        case HSS_ERR_INVALID_VARIANT: return "invalid arithmetic variant";
+        case HSS_ERR_NO_DEVICE: return "no HIP device found";
+        default:
+            if (status > 0) return hipGetErrorString((hipError_t)status);
+            return "unknown status";
+    }
+}
+
+int hss_device_count(int *count) {
+    if (!count) return HSS_ERR_NULL_POINTER;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return e == hipErrorNoDevice ? HSS_ERR_NO_DEVICE : (int)e;
+    }
+    *count = n;
+    return n > 0 ? HSS_OK : HSS_ERR_NO_DEVICE;
+}
+
+int hss_build_parameter_array(int model, const float *lower, const float *step,
+                              const int32_t *count, float *d_params, void *stream) {
+    const int np = model_param_count(model);
+    if (np < 0) return HSS_ERR_INVALID_MODEL;
+    if (!lower || !step || !count || !d_params) return HSS_ERR_NULL_POINTER;
+    ParamGridArgs a{};
+    unsigned long long total = 1;
+    for (int d = 0; d < 3; ++d) {
+        a.lo[d] = d < np ? lower[d] : 0.0f;
+        a.step[d] = d < np ? step[d] : 0.0f;
+        a.cnt[d] = d < np ? (unsigned)count[d] : 1u;
+        if (d < np && count[d] <= 0) return HSS_ERR_INVALID_SIZE;
+        total *= a.cnt[d];
+    }
+    if (total > 0x7fffffffULL) return HSS_ERR_INVALID_SIZE;
+    a.total = (unsigned)total;
+    const unsigned threads = 256, blocks = (a.total + threads - 1) / threads;
+    hipStream_t st = (hipStream_t)stream;
+    dispatch_model(model, [&](auto M) {
+        hipLaunchKernelGGL(build_parameter_array_kernel<decltype(M)::value>, dim3(blocks),
+                           dim3(threads), 0, st, a, d_params);
+    });
+    return hip_status(hipGetLastError());
+}
+
+int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int model,
+                       const float *d_params, int32_t P, int variant, float *d_result,
+                       float *d_sembl, float *d_stack, unsigned long long *d_not_used,
+                       void *stream) {
+    int rc = check_traces(traces);
+    if (rc != HSS_OK) return rc;
+    rc = check_midpoints(mids);
+    if (rc != HSS_OK) return rc;
+    if (model_param_count(model) < 0) return HSS_ERR_INVALID_MODEL;
+    if (!d_params || !d_result) return HSS_ERR_NULL_POINTER;
+    if (P <= 0) return HSS_ERR_INVALID_SIZE;
+    if (variant != HSS_EXACT && variant != HSS_FAST) return HSS_ERR_INVALID_VARIANT;
+    if (mids->n == 0) return HSS_OK;
+
+    GridArgs a{};
+    a.samples = traces->samples;
+    a.scale = traces->scale;
+    a.mid = traces->mid;
+    a.hoff = traces->hoff;
+    a.hoff2 = traces->hoff2;
+    a.ns = traces->ns;
+    a.dt = traces->dt;
+    a.tau = traces->tau;
+    a.w = traces->w;
+    a.mbeg = mids->begin;
+    a.mcnt = mids->count;
+    a.mm0 = mids->m0;
+    a.midx = mids->index;
+    a.h0 = mids->h0;
+    a.apm = mids->apm;
+    a.params = d_params;
+    a.P = P;
+    a.ntiles = (traces->ns + kWave - 1) / kWave;
+    a.result = d_result;
+    a.sembl = d_sembl;
+    a.stack = d_stack;
+    a.not_used = d_not_used;
+    if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
+        return HSS_ERR_INVALID_SIZE;
+    hipStream_t st = (hipStream_t)stream;
+    switch (model) {
+        case HSS_CMP: return launch_grid_cmp(a, mids->n, variant, traces->storage, st);
+        case HSS_ZOCRS: return launch_grid_zocrs(a, mids->n, variant, traces->storage, st);
+        case HSS_OCT: return launch_grid_oct(a, mids->n, variant, traces->storage, st);
+        default: return HSS_ERR_INVALID_MODEL;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// Device-side building blocks of the semblance hot path (gfx950).
+//
+// Every translation unit that includes this header is compiled with
+// -ffp-contract=off: a float expression is evaluated as written, one IEEE-rounded
+// operation per operator, in the order of the reference source text.  Where a fast
+// variant wants a fused multiply-add it calls __builtin_fmaf explicitly.
+// sqrtf and '/' are the correctly rounded HIP defaults.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/hss.h"
+
+namespace hss {
+
+constexpr int kWave = 64;
+constexpr int kMaxWindow = 32;  // MAX_WINDOW_SIZE, common/include/gpu/interface.h:10
+
+enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
+    kNoError = 0,
+    kDivisionByZero = 1,
+    kNegativeSquaredRoot = 2,
+};
+
+// ---------------------------------------------------------------------------
+// OCT displaced midpoint (reference: cuda/include/semblance/kernel/oct/common.cuh:7-35).
+__device__ __forceinline__ int oct_displaced_midpoint(float h, float h0, float t0, float m0,
+                                                      float c, float slope, float *mh) {
+    float theta = t0 * slope;
+    float gamma = 2.0f * sqrtf(h * h + h0 * h0);
+    float tn0_sq = t0 * t0 - c * h0 * h0;
+    float theta_sq = theta * theta;
+    float gamma_sq = gamma * gamma;
+    float tn0_quad = tn0_sq * tn0_sq;
+    float arg = tn0_quad * tn0_quad + tn0_quad * theta_sq * gamma_sq +
+                16.0f * h * h * h0 * h0 * theta_sq * theta_sq;
+    if (arg < 0) return kNegativeSquaredRoot;
+    arg = theta_sq * gamma_sq + 2.0f * tn0_quad + 2.0f * sqrtf(arg);
+    if (arg == 0) return kDivisionByZero;
+    *mh = m0 + 2.0f * theta * (h * h - h0 * h0) / sqrtf(arg);
+    return kNoError;
+}
+
+// OCT traveltime along the trajectory (reference: oct/common.cuh:37-94).
+__device__ __forceinline__ int oct_time(float h, float h0, float t0, float m0, float m, float mh,
+                                        float c, float slope, float *out) {
+    float tn0_sq = t0 * t0 - c * h0 * h0;
+    float w1 = (h + h0) * (h + h0) - (mh - m0) * (mh - m0);
+    float w2 = (h - h0) * (h - h0) - (mh - m0) * (mh - m0);
+    if (w1 < 0 || w2 < 0) return kNegativeSquaredRoot;
+    float u = sqrtf(w1) + sqrtf(w2);
+    float th = t0;
+    if (fabsf(h) > fabsf(h0)) {
+        if (u == 0.0f) return kDivisionByZero;
+        float arg = c * h * h + 4.0f * h * h / (u * u) * tn0_sq;
+        if (arg < 0) return kNegativeSquaredRoot;
+        th = sqrtf(arg);
+    } else if (fabsf(h) < fabsf(h0)) {
+        if (h0 == 0.0f) return kDivisionByZero;
+        float arg = c * h * h + u * u / (4.0f * h0 * h0) * tn0_sq;
+        if (arg < 0) return kNegativeSquaredRoot;
+        th = sqrtf(arg);
+    }
+    float tn_sq = th * th - c * h * h;
+    if (th == 0.0f || tn0_sq == 0.0f) return kDivisionByZero;
+    float ah = (t0 * tn_sq) / (th * tn0_sq) * slope;
+    *out = th + ah * (m - mh);
+    return kNoError;
+}
+
+// ---------------------------------------------------------------------------
+// A candidate in storage form: CMP {c}; ZO-CRS {c, A, B}; OCT {c, slope}.
+struct Candidate {
+    float c, p1, p2;
+};
+
+// Per-trace geometry a traveltime needs.
+struct TraceGeom {
+    float m, h, h2;
+};
+
+// Traveltime of one trace for one candidate at zero-offset time t0.  Returns true when
+// the trace enters the window sum; sets *reject when the model rejects the trace the way
+// the reference counts in notUsedCount (OCT only).
+//   CMP    cmp/linear_search.cu:60-62
+//   ZO-CRS zocrs/linear_search.cu:83-90
+//   OCT    oct/linear_search.cu:84-98
+template <int MODEL>
+__device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float m0, float h0,
+                                           float apm, const Candidate &k, float *t, bool *reject) {
+    *reject = false;
+    if constexpr (MODEL == HSS_CMP) {
+        *t = sqrtf(t0 * t0 + k.c * tg.h2);
+        return true;
+    } else if constexpr (MODEL == HSS_ZOCRS) {
+        float dm = tg.m - m0;
+        float tmp = t0 + k.p1 * dm;
+        tmp = tmp * tmp + k.p2 * dm * dm + k.c * tg.h2;
+        if (tmp >= 0) {
+            *t = sqrtf(tmp);
+            return true;
+        }
+        return false;
+    } else {
+        float mh;
+        int err = oct_displaced_midpoint(tg.h, h0, t0, m0, k.c, k.p1, &mh);
+        if (err != kNoError || fabsf(tg.m - mh) > apm) {
+            *reject = true;
+            return false;
+        }
+        err = oct_time(tg.h, h0, t0, m0, tg.m, mh, k.c, k.p1, t);
+        return err == kNoError;
+    }
+}
+
+// Sample index and interpolation weight (COMPUTE_SEMBLANCE, interface.h:27-31):
+// tIndex = t/dt, k = (int)tIndex, frac = tIndex - k, trace kept when
+// k - tau >= 0 && k + tau + 1 < ns.  A NaN, infinite or out-of-int-range index drops
+// the trace (the reference's cast is undefined there); for every other value the
+// decision is the reference's.  On success *first = k - tau.
+__device__ __forceinline__ bool window_index(float t, float dt, int ns, int tau, int *first,
+                                             float *frac) {
+    float ti = t / dt;
+    if (!(ti > -1.0f && ti < 2.0e9f)) return false;
+    int k = (int)ti;
+    *frac = ti - (float)k;
+    *first = k - tau;
+    return (k - tau >= 0) && (k + tau + 1 < ns);
+}
+
+// ---------------------------------------------------------------------------
+// Sample fetch for each storage type.  F16/BF16 values are widened to float and
+// multiplied by the trace's scale; F32 is read as is (times scale when one is given).
+template <int STORAGE>
+struct SampleReader;
+
+template <>
+struct SampleReader<HSS_F32> {
+    const float *__restrict__ p;
+    __device__ __forceinline__ float operator()(size_t i) const { return p[i]; }
+};
+
+template <>
+struct SampleReader<HSS_F16> {
+    const _Float16 *__restrict__ p;
+    __device__ __forceinline__ float operator()(size_t i) const { return (float)p[i]; }
+};
+
+template <>
+struct SampleReader<HSS_BF16> {
+    const uint16_t *__restrict__ p;
+    __device__ __forceinline__ float operator()(size_t i) const {
+        return __uint_as_float((uint32_t)p[i] << 16);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers.
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+}  // namespace hss

@@ -1,0 +1,8 @@
+// Grid-search semblance instantiations for the CMP traveltime (see grid_impl.hpp).
+#include "grid_impl.hpp"
+
+namespace hss {
+int launch_grid_cmp(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+    return launch_grid<HSS_CMP>(a, n_mid, variant, storage, stream);
+}
+}  // namespace hss

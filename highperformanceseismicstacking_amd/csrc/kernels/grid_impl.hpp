@@ -1,0 +1,205 @@
+// Grid ("linear") search semblance with the best-candidate pick fused in.
+//
+// Reference behaviour: computeSemblancesFor<Model> (cmp/linear_search.cu:19-81,
+// zocrs/linear_search.cu:36-109, oct/linear_search.cu:30-114) evaluates one thread per
+// (t0, candidate), and selectBestSemblancesFor<Model> (cmp/linear_search.cu:83-115 and
+// twins) scans the candidates of each t0 for the first strict maximum.
+//
+// MI355X mapping: one workgroup = (midpoint, tile of 64 consecutive t0 samples), four
+// wavefronts.  Lane l of every wave owns t0 sample tile*64 + l; wave v evaluates
+// candidates v, v+4, v+8, ... in increasing order.  The candidate is wave-uniform (its
+// parameters sit in SGPRs), each lane walks the midpoint's traces in order and reads a
+// (w+1)-sample window that is nearly contiguous across the 64 lanes, so the loads
+// coalesce.  Each lane keeps its running best (strict '>', so the lowest index wins
+// among equal values and NaN never wins); the four per-wave bests are merged through LDS
+// (larger value, then lower index), which reproduces the reference's sequential scan.
+#pragma once
+
+#include "launch_util.hpp"
+#include "window.hpp"
+
+namespace hss {
+
+constexpr int kGridWaves = 4;
+constexpr int kGridThreads = kGridWaves * kWave;
+
+struct GridArgs {
+    const void *samples;
+    const float *scale;
+    const float *mid;
+    const float *hoff;
+    const float *hoff2;
+    int ns;
+    float dt;
+    int tau;
+    int w;
+    const int *mbeg;
+    const int *mcnt;
+    const float *mm0;
+    const int *midx;
+    float h0;
+    float apm;
+    const float *params;
+    int P;
+    int ntiles;
+    float *result;
+    float *sembl;
+    float *stack;
+    unsigned long long *not_used;
+};
+
+template <int MODEL>
+__device__ __forceinline__ Candidate load_candidate(const float *__restrict__ params, int q) {
+    Candidate k{0.0f, 0.0f, 0.0f};
+    if constexpr (MODEL == HSS_CMP) {
+        k.c = params[q];
+    } else if constexpr (MODEL == HSS_ZOCRS) {
+        k.c = params[3 * q];
+        k.p1 = params[3 * q + 1];
+        k.p2 = params[3 * q + 2];
+    } else {
+        k.c = params[2 * q];
+        k.p1 = params[2 * q + 1];
+    }
+    return k;
+}
+
+template <int MODEL>
+__device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
+                                               const float *__restrict__ hoff,
+                                               const float *__restrict__ hoff2, int ti) {
+    TraceGeom g{0.0f, 0.0f, 0.0f};
+    if constexpr (MODEL == HSS_CMP) {
+        g.h2 = hoff2[ti];
+    } else if constexpr (MODEL == HSS_ZOCRS) {
+        g.m = mid[ti];
+        g.h2 = hoff2[ti];
+    } else {
+        g.m = mid[ti];
+        g.h = hoff[ti];
+    }
+    return g;
+}
+
+template <int MODEL, int W, int VARIANT, int STORAGE>
+__global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a) {
+    using Reader = SampleReader<STORAGE>;
+    constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
+    __shared__ float sh_best[kGridWaves][kWave];
+    __shared__ float sh_stack[kGridWaves][kWave];
+    __shared__ int sh_idx[kGridWaves][kWave];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int g = blockIdx.x / a.ntiles;
+    const int tile = blockIdx.x - g * a.ntiles;
+    const int s = tile * kWave + lane;
+    const bool active = s < a.ns;
+    const float t0 = (float)s * a.dt;
+
+    const int beg = a.mbeg[g];
+    const int cnt = a.mcnt[g];
+    const float m0 = a.mm0[g];
+    const bool scaled = (STORAGE != HSS_F32) && a.scale != nullptr;
+    Reader rd{(decltype(Reader::p))a.samples};
+
+    float best = -1.0f, bstack = 0.0f;
+    int bidx = -1;
+    unsigned long long rejected = 0;
+
+    for (int q = wave; q < a.P; q += kGridWaves) {
+        const Candidate k = load_candidate<MODEL>(a.params, q);
+        Window<W, VARIANT> acc;
+        acc.reset();
+        unsigned rej_q = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const int ti = a.midx ? a.midx[beg + j] : beg + j;
+            const TraceGeom tg = load_geom<MODEL>(a.mid, a.hoff, a.hoff2, ti);
+            float t;
+            bool rej;
+            bool ok = traveltime<MODEL>(tg, t0, m0, a.h0, a.apm, k, &t, &rej);
+            rej_q += rej ? 1u : 0u;
+            int first;
+            float f;
+            ok = ok && window_index(t, a.dt, a.ns, a.tau, &first, &f);
+            if (ok && active) {
+                const float sc = scaled ? a.scale[ti] : 1.0f;
+                acc.add(rd, (size_t)ti * (size_t)a.ns + (size_t)first, f, sc, scaled, a.w);
+            }
+        }
+        float S, stk;
+        acc.reduce(a.w, &S, &stk);
+        if (active) {
+            rejected += rej_q;
+            if (a.sembl) a.sembl[((size_t)g * a.ns + s) * a.P + q] = S;
+            if (a.stack) a.stack[((size_t)g * a.ns + s) * a.P + q] = stk;
+            if (S > best) {
+                best = S;
+                bstack = stk;
+                bidx = q;
+            }
+        }
+    }
+
+    if (MODEL == HSS_OCT && a.not_used) {
+        unsigned long long tot = wave_sum_u64(rejected);
+        if (lane == 0 && tot) atomicAdd(a.not_used + g, tot);
+    }
+
+    sh_best[wave][lane] = best;
+    sh_stack[wave][lane] = bstack;
+    sh_idx[wave][lane] = bidx;
+    __syncthreads();
+    if (wave != 0 || !active) return;
+
+    float B = -1.0f, BS = 0.0f;
+    int BI = -1;
+#pragma unroll
+    for (int v = 0; v < kGridWaves; ++v) {
+        const int iv = sh_idx[v][lane];
+        const float sv = sh_best[v][lane];
+        if (iv >= 0 && (sv > B || (sv == B && iv < BI))) {
+            B = sv;
+            BS = sh_stack[v][lane];
+            BI = iv;
+        }
+    }
+    float *res = a.result + (size_t)g * (2 + NP) * a.ns;
+    res[s] = B;
+    res[a.ns + s] = BI >= 0 ? BS : 0.0f;
+    if (BI >= 0) {
+        const Candidate k = load_candidate<MODEL>(a.params, BI);
+        res[2 * a.ns + s] = 2 / sqrtf(k.c);
+        if constexpr (NP >= 2) res[3 * a.ns + s] = k.p1;
+        if constexpr (NP >= 3) res[4 * a.ns + s] = k.p2;
+    } else {
+#pragma unroll
+        for (int d = 0; d < NP; ++d) res[(2 + d) * a.ns + s] = 0.0f;
+    }
+}
+
+// Host launcher for one model: picks the window/variant/storage instantiation.
+template <int MODEL>
+int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+    int st = HSS_OK;
+    const unsigned blocks = (unsigned)n_mid * (unsigned)a.ntiles;
+    if (blocks == 0) return HSS_OK;
+    int rc = dispatch_storage(storage, [&](auto S) {
+        st = dispatch_variant(variant, [&](auto V) {
+            dispatch_window(a.w, [&](auto Wc) {
+                hipLaunchKernelGGL((grid_semblance_kernel<MODEL, decltype(Wc)::value,
+                                                          decltype(V)::value, decltype(S)::value>),
+                                   dim3(blocks), dim3(kGridThreads), 0, stream, a);
+            });
+        });
+    });
+    if (rc != HSS_OK) return rc;
+    if (st != HSS_OK) return st;
+    return hip_status(hipGetLastError());
+}
+
+int launch_grid_cmp(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_grid_zocrs(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+
+}  // namespace hss

@@ -1,0 +1,8 @@
+// Grid-search semblance instantiations for the OCT traveltime (see grid_impl.hpp).
+#include "grid_impl.hpp"
+
+namespace hss {
+int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+    return launch_grid<HSS_OCT>(a, n_mid, variant, storage, stream);
+}
+}  // namespace hss

@@ -1,0 +1,82 @@
+// Host-side helpers shared by the C-ABI entry points: argument validation and the
+// compile-time window-size dispatch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../../include/hss.h"
+
+namespace hss {
+
+inline int model_param_count(int model) {
+    switch (model) {
+        case HSS_CMP: return 1;
+        case HSS_ZOCRS: return 3;
+        case HSS_OCT: return 2;
+        default: return -1;
+    }
+}
+
+inline int check_traces(const hss_traces *t) {
+    if (!t || !t->samples || !t->mid || !t->hoff || !t->hoff2) return HSS_ERR_NULL_POINTER;
+    if (t->ntr < 0 || t->ns <= 0) return HSS_ERR_INVALID_SIZE;
+    if (t->tau < 0 || t->w != 2 * t->tau + 1 || t->w > 32) return HSS_ERR_INVALID_WINDOW;
+    if (t->storage < HSS_F32 || t->storage > HSS_BF16) return HSS_ERR_INVALID_STORAGE;
+    if (!(t->dt > 0.0f)) return HSS_ERR_INVALID_SIZE;
+    return HSS_OK;
+}
+
+inline int check_midpoints(const hss_midpoints *m) {
+    if (!m) return HSS_ERR_NULL_POINTER;
+    if (m->n < 0) return HSS_ERR_INVALID_SIZE;
+    if (m->n > 0 && (!m->begin || !m->count || !m->m0)) return HSS_ERR_NULL_POINTER;
+    return HSS_OK;
+}
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; }
+
+// Calls f(std::integral_constant<int, W>) for the compiled window sizes; any other w
+// (odd, <= 32) goes to the runtime-w instantiation W = 0.
+template <class F>
+inline void dispatch_window(int w, F &&f) {
+    switch (w) {
+        case 5: f(std::integral_constant<int, 5>{}); break;
+        case 7: f(std::integral_constant<int, 7>{}); break;
+        case 9: f(std::integral_constant<int, 9>{}); break;
+        case 11: f(std::integral_constant<int, 11>{}); break;
+        case 13: f(std::integral_constant<int, 13>{}); break;
+        case 15: f(std::integral_constant<int, 15>{}); break;
+        default: f(std::integral_constant<int, 0>{}); break;
+    }
+}
+
+template <class F>
+inline int dispatch_model(int model, F &&f) {
+    switch (model) {
+        case HSS_CMP: f(std::integral_constant<int, HSS_CMP>{}); return HSS_OK;
+        case HSS_ZOCRS: f(std::integral_constant<int, HSS_ZOCRS>{}); return HSS_OK;
+        case HSS_OCT: f(std::integral_constant<int, HSS_OCT>{}); return HSS_OK;
+        default: return HSS_ERR_INVALID_MODEL;
+    }
+}
+
+template <class F>
+inline int dispatch_storage(int storage, F &&f) {
+    switch (storage) {
+        case HSS_F32: f(std::integral_constant<int, HSS_F32>{}); return HSS_OK;
+        case HSS_F16: f(std::integral_constant<int, HSS_F16>{}); return HSS_OK;
+        case HSS_BF16: f(std::integral_constant<int, HSS_BF16>{}); return HSS_OK;
+        default: return HSS_ERR_INVALID_STORAGE;
+    }
+}
+
+template <class F>
+inline int dispatch_variant(int variant, F &&f) {
+    switch (variant) {
+        case HSS_EXACT: f(std::integral_constant<int, HSS_EXACT>{}); return HSS_OK;
+        case HSS_FAST: f(std::integral_constant<int, HSS_FAST>{}); return HSS_OK;
+        default: return HSS_ERR_INVALID_VARIANT;
+    }
+}
+
+}  // namespace hss
