@@ -32,6 +32,9 @@ def parse():
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU work budget of the oracle baseline sample (0 disables it)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="after the timed steps, run one plain device-to-device copy of this "
+                         "many MiB (a known-size reference for rocprofv3 FETCH/WRITE counters)")
     return ap.parse_args()
 
 
@@ -121,6 +124,15 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+
+    if args.calibrate_copy_mb > 0:
+        nbytes = args.calibrate_copy_mb << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     n_evals = int(mp.n) * line.ns * G
     sample_bytes = dl.sample_bytes()

@@ -46,6 +46,23 @@ class hss_midpoints(ctypes.Structure):
     ]
 
 
+class hss_oct_candidates(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("d_pairs", ctypes.c_void_p),
+        ("npairs", ctypes.c_int32),
+        ("d_vparams", ctypes.c_void_p),
+        ("NP", ctypes.c_int32),
+        ("lower", ctypes.c_float * 2),
+        ("upper", ctypes.c_float * 2),
+        ("seed", ctypes.c_uint64),
+        ("stream_base", ctypes.c_uint32),
+    ]
+
+
+OCT_PAIRS, OCT_PER_SAMPLE, OCT_DE_START = 0, 1, 2
+
+
 def lib():
     """The loaded libhss.so (raises HssError when it is not built)."""
     global _lib
@@ -54,14 +71,20 @@ def lib():
             raise HssError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
-        L.hss_abi_version.restype = ctypes.c_int
+        ci, u64, u32 = ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32
+        tr, mp = ctypes.POINTER(hss_traces), ctypes.POINTER(hss_midpoints)
+        L.hss_abi_version.restype = ci
         L.hss_status_string.restype = ctypes.c_char_p
-        L.hss_status_string.argtypes = [ctypes.c_int]
-        L.hss_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
-        L.hss_build_parameter_array.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp]
-        L.hss_grid_semblance.argtypes = [
-            ctypes.POINTER(hss_traces), ctypes.POINTER(hss_midpoints), ctypes.c_int,
-            vp, i32, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.hss_status_string.argtypes = [ci]
+        L.hss_device_count.argtypes = [ctypes.POINTER(ci)]
+        L.hss_build_parameter_array.argtypes = [ci, vp, vp, vp, vp, vp]
+        L.hss_grid_semblance.argtypes = [tr, mp, ci, vp, i32, ci, vp, vp, vp, vp, vp]
+        L.hss_de_workspace_size.argtypes = [i32, i32, ci, i32, ctypes.POINTER(u64)]
+        L.hss_de_semblance.argtypes = [tr, mp, ci, vp, vp, i32, i32, u64, u32, ci, vp, u64,
+                                       vp, vp, vp, vp]
+        L.hss_stretch_free.argtypes = [tr, mp, ci, vp, vp, i32, ci, vp, vp, vp, vp, vp]
+        L.hss_select_traces.argtypes = [tr, ci, i32, vp, f32, f32,
+                                        ctypes.POINTER(hss_oct_candidates), vp, vp]
         _lib = L
     return _lib
 
@@ -220,3 +243,93 @@ def grid_semblance(line, mids, model, params, P, variant=FAST, full=False,
                                    _ptr(params), int(P), int(variant), _ptr(result),
                                    _ptr(sembl), _ptr(stack), _ptr(nu), _stream_handle(stream)))
     return result, sembl, stack, nu
+
+
+def de_workspace_bytes(n_mid, ns, model, NP):
+    out = ctypes.c_uint64(0)
+    check(lib().hss_de_workspace_size(int(n_mid), int(ns), int(model), int(NP), ctypes.byref(out)))
+    return int(out.value)
+
+
+def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, variant=FAST,
+                 population=False, result=None, workspace=None, stream=None):
+    """Returns (result [n,(2+np),ns], pop_x [n,ns,NP,np] or None, pop_fx [n,ns,NP,2] or None)."""
+    import torch
+    check_ranges(line, mids)
+    dev = line.samples.device
+    np_ = MODEL_NPARAMS[model]
+    lo = (ctypes.c_float * 3)(*[float(x) for x in list(lower)[:np_]] + [0.0] * (3 - np_))
+    hi = (ctypes.c_float * 3)(*[float(x) for x in list(upper)[:np_]] + [0.0] * (3 - np_))
+    nbytes = de_workspace_bytes(mids.n, line.ns, model, NP)
+    if workspace is None:
+        workspace = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device=dev)
+    if result is None:
+        result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32, device=dev)
+    px = pfx = None
+    if population:
+        px = torch.empty((mids.n, line.ns, NP, np_), dtype=torch.float32, device=dev)
+        pfx = torch.empty((mids.n, line.ns, NP, 2), dtype=torch.float32, device=dev)
+    check(lib().hss_de_semblance(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
+                                 ctypes.cast(lo, ctypes.c_void_p), ctypes.cast(hi, ctypes.c_void_p),
+                                 int(NP), int(G), ctypes.c_uint64(seed), ctypes.c_uint32(stream_base),
+                                 int(variant), _ptr(workspace), ctypes.c_uint64(workspace.numel() * 4),
+                                 _ptr(result), _ptr(px), _ptr(pfx), _stream_handle(stream)))
+    return result, px, pfx
+
+
+def stretch_free(line, mids, model, vparams, nvals, variant=FAST, full=False, not_used=False,
+                 stream=None):
+    """vparams: device tensor [n, np, ns] (velocity, then A,B or slope).
+    Returns (result [n,3,ns], sembl [n,ns,nN] or None, stack or None, not_used or None)."""
+    import torch
+    check_ranges(line, mids)
+    dev = line.samples.device
+    np_ = MODEL_NPARAMS[model]
+    if vparams.numel() < mids.n * np_ * line.ns:
+        raise ValueError("vparams shorter than [n, np, ns]")
+    nv = np.ascontiguousarray(nvals, dtype=np.float32)
+    nN = nv.size
+    result = torch.empty((mids.n, 3, line.ns), dtype=torch.float32, device=dev)
+    sembl = stack = nu = None
+    if full:
+        sembl = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
+        stack = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
+    if not_used:
+        nu = torch.zeros(mids.n, dtype=torch.int64, device=dev)
+    check(lib().hss_stretch_free(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
+                                 _ptr(vparams), nv.ctypes.data_as(ctypes.c_void_p), int(nN),
+                                 int(variant), _ptr(result), _ptr(sembl), _ptr(stack), _ptr(nu),
+                                 _stream_handle(stream)))
+    return result, sembl, stack, nu
+
+
+def select_traces(line, model, m0, apm=0.0, h0=0.0, oct_mode=None, pairs=None, vparams=None,
+                  NP=0, lower=(0.0, 0.0), upper=(0.0, 0.0), seed=0, stream_base=0, stream=None):
+    """Trace masks [n, ntr] (uint8 tensor on the device) over the whole line."""
+    import torch
+    dev = line.samples.device
+    m0 = np.ascontiguousarray(m0, dtype=np.float32)
+    dm0 = torch.from_numpy(m0).to(dev)
+    mask = torch.zeros((m0.size, line.ntr), dtype=torch.uint8, device=dev)
+    octc = None
+    if model == 2:
+        octc = hss_oct_candidates()
+        octc.mode = oct_mode
+        if oct_mode == OCT_PAIRS:
+            octc.d_pairs = pairs.data_ptr()
+            octc.npairs = pairs.numel() // 2
+        elif oct_mode == OCT_PER_SAMPLE:
+            if vparams.numel() < m0.size * 2 * line.ns:
+                raise ValueError("vparams shorter than [n, 2, ns]")
+            octc.d_vparams = vparams.data_ptr()
+        else:
+            octc.NP = int(NP)
+            octc.lower[0], octc.lower[1] = float(lower[0]), float(lower[1])
+            octc.upper[0], octc.upper[1] = float(upper[0]), float(upper[1])
+            octc.seed = int(seed)
+            octc.stream_base = int(stream_base)
+    check(lib().hss_select_traces(ctypes.byref(line.struct), int(model), int(m0.size), _ptr(dm0),
+                                  float(apm), float(h0),
+                                  ctypes.byref(octc) if octc is not None else None,
+                                  _ptr(mask), _stream_handle(stream)))
+    return mask

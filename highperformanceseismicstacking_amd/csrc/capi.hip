@@ -1,9 +1,12 @@
-// extern "C" entry points of libhss.so (declared in include/hss.h).
+// extern "C" entry points of libhss.so (declared in include/hss.h), except
+// hss_select_traces (kernels/select.hip).
 #include <hip/hip_runtime.h>
 
 #include "../../include/hss.h"
+#include "kernels/de_impl.hpp"
 #include "kernels/grid_impl.hpp"
 #include "kernels/launch_util.hpp"
+#include "kernels/sf_impl.hpp"
 
 namespace hss {
 
@@ -40,6 +43,37 @@ __global__ void build_parameter_array_kernel(ParamGridArgs a, float *out) {
     }
 }
 
+// The midpoint-independent part of an evaluation context.
+static EvalCtx base_ctx(const hss_traces *t, const hss_midpoints *m) {
+    EvalCtx e{};
+    e.samples = t->samples;
+    e.scale = t->scale;
+    e.mid = t->mid;
+    e.hoff = t->hoff;
+    e.hoff2 = t->hoff2;
+    e.midx = m->index;
+    e.ns = t->ns;
+    e.tau = t->tau;
+    e.w = t->w;
+    e.dt = t->dt;
+    e.h0 = m->h0;
+    e.apm = m->apm;
+    return e;
+}
+
+static int check_batch(const hss_traces *traces, const hss_midpoints *mids, int model,
+                       int variant) {
+    int rc = check_traces(traces);
+    if (rc != HSS_OK) return rc;
+    rc = check_midpoints(mids);
+    if (rc != HSS_OK) return rc;
+    if (model_param_count(model) < 0) return HSS_ERR_INVALID_MODEL;
+    if (variant != HSS_EXACT && variant != HSS_FAST) return HSS_ERR_INVALID_VARIANT;
+    return HSS_OK;
+}
+
+static int tiles_of(int ns) { return (ns + kWave - 1) / kWave; }
+
 }  // namespace hss
 
 using namespace hss;
@@ -51,7 +85,7 @@ int hss_abi_version(void) { return HSS_ABI_VERSION; }
 const char *hss_status_string(int status) {
     switch (status) {
         case HSS_OK: return "success";
-        case HSS_ERR_INVALID_MODEL: return "invalid traveltime model";
+        case HSS_ERR_INVALID_MODEL: return "invalid traveltime model or mode";
         case HSS_ERR_INVALID_WINDOW: return "invalid window: w must be 2*tau+1 and at most 32";
         case HSS_ERR_INVALID_SIZE: return "invalid size";
         case HSS_ERR_NULL_POINTER: return "required pointer is NULL";
@@ -105,35 +139,19 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
                        const float *d_params, int32_t P, int variant, float *d_result,
                        float *d_sembl, float *d_stack, unsigned long long *d_not_used,
                        void *stream) {
-    int rc = check_traces(traces);
+    int rc = check_batch(traces, mids, model, variant);
     if (rc != HSS_OK) return rc;
-    rc = check_midpoints(mids);
-    if (rc != HSS_OK) return rc;
-    if (model_param_count(model) < 0) return HSS_ERR_INVALID_MODEL;
     if (!d_params || !d_result) return HSS_ERR_NULL_POINTER;
     if (P <= 0) return HSS_ERR_INVALID_SIZE;
-    if (variant != HSS_EXACT && variant != HSS_FAST) return HSS_ERR_INVALID_VARIANT;
     if (mids->n == 0) return HSS_OK;
-
     GridArgs a{};
-    a.samples = traces->samples;
-    a.scale = traces->scale;
-    a.mid = traces->mid;
-    a.hoff = traces->hoff;
-    a.hoff2 = traces->hoff2;
-    a.ns = traces->ns;
-    a.dt = traces->dt;
-    a.tau = traces->tau;
-    a.w = traces->w;
+    a.e = base_ctx(traces, mids);
     a.mbeg = mids->begin;
     a.mcnt = mids->count;
     a.mm0 = mids->m0;
-    a.midx = mids->index;
-    a.h0 = mids->h0;
-    a.apm = mids->apm;
     a.params = d_params;
     a.P = P;
-    a.ntiles = (traces->ns + kWave - 1) / kWave;
+    a.ntiles = tiles_of(traces->ns);
     a.result = d_result;
     a.sembl = d_sembl;
     a.stack = d_stack;
@@ -144,8 +162,93 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     switch (model) {
         case HSS_CMP: return launch_grid_cmp(a, mids->n, variant, traces->storage, st);
         case HSS_ZOCRS: return launch_grid_zocrs(a, mids->n, variant, traces->storage, st);
-        case HSS_OCT: return launch_grid_oct(a, mids->n, variant, traces->storage, st);
-        default: return HSS_ERR_INVALID_MODEL;
+        default: return launch_grid_oct(a, mids->n, variant, traces->storage, st);
+    }
+}
+
+int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint64_t *bytes) {
+    const int np = model_param_count(model);
+    if (np < 0) return HSS_ERR_INVALID_MODEL;
+    if (!bytes) return HSS_ERR_NULL_POINTER;
+    if (n_mid < 0 || ns <= 0 || NP <= 0) return HSS_ERR_INVALID_SIZE;
+    *bytes = (uint64_t)n_mid * (uint64_t)tiles_of(ns) * 2ull * (uint64_t)NP * (uint64_t)(np + 2) *
+             (uint64_t)kWave * sizeof(float);
+    return HSS_OK;
+}
+
+int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int model,
+                     const float *lower, const float *upper, int32_t NP, int32_t G,
+                     uint64_t seed, uint32_t stream_base, int variant, void *d_workspace,
+                     uint64_t workspace_bytes, float *d_result, float *d_pop_x, float *d_pop_fx,
+                     void *stream) {
+    int rc = check_batch(traces, mids, model, variant);
+    if (rc != HSS_OK) return rc;
+    if (!lower || !upper || !d_result || !d_workspace) return HSS_ERR_NULL_POINTER;
+    if (NP < 1 || G < 1) return HSS_ERR_INVALID_SIZE;
+    uint64_t need = 0;
+    rc = hss_de_workspace_size(mids->n, traces->ns, model, NP, &need);
+    if (rc != HSS_OK) return rc;
+    if (workspace_bytes < need) return HSS_ERR_INVALID_SIZE;
+    if (mids->n == 0) return HSS_OK;
+    DeArgs a{};
+    a.e = base_ctx(traces, mids);
+    a.mbeg = mids->begin;
+    a.mcnt = mids->count;
+    a.mm0 = mids->m0;
+    a.NP = NP;
+    a.G = G;
+    const int np = model_param_count(model);
+    for (int d = 0; d < 3; ++d) {
+        a.lo[d] = d < np ? lower[d] : 0.0f;
+        a.hi[d] = d < np ? upper[d] : 0.0f;
+    }
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.stream_base = stream_base;
+    a.ntiles = tiles_of(traces->ns);
+    a.ws = (float *)d_workspace;
+    a.result = d_result;
+    a.pop_x = d_pop_x;
+    a.pop_fx = d_pop_fx;
+    if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
+        return HSS_ERR_INVALID_SIZE;
+    hipStream_t st = (hipStream_t)stream;
+    switch (model) {
+        case HSS_CMP: return launch_de_cmp(a, mids->n, variant, traces->storage, st);
+        case HSS_ZOCRS: return launch_de_zocrs(a, mids->n, variant, traces->storage, st);
+        default: return launch_de_oct(a, mids->n, variant, traces->storage, st);
+    }
+}
+
+int hss_stretch_free(const hss_traces *traces, const hss_midpoints *mids, int model,
+                     const float *d_vparams, const float *nvals, int32_t nN, int variant,
+                     float *d_result, float *d_sembl, float *d_stack,
+                     unsigned long long *d_not_used, void *stream) {
+    int rc = check_batch(traces, mids, model, variant);
+    if (rc != HSS_OK) return rc;
+    if (!d_vparams || !nvals || !d_result) return HSS_ERR_NULL_POINTER;
+    if (nN <= 0 || nN > kSfMaxShifts) return HSS_ERR_INVALID_SIZE;
+    if (mids->n == 0) return HSS_OK;
+    SfArgs a{};
+    a.e = base_ctx(traces, mids);
+    a.mbeg = mids->begin;
+    a.mcnt = mids->count;
+    a.mm0 = mids->m0;
+    a.vparams = d_vparams;
+    for (int q = 0; q < nN; ++q) a.nvals[q] = nvals[q];
+    a.nN = nN;
+    a.ntiles = tiles_of(traces->ns);
+    a.result = d_result;
+    a.sembl = d_sembl;
+    a.stack = d_stack;
+    a.not_used = d_not_used;
+    if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
+        return HSS_ERR_INVALID_SIZE;
+    hipStream_t st = (hipStream_t)stream;
+    switch (model) {
+        case HSS_CMP: return launch_sf_cmp(a, mids->n, variant, traces->storage, st);
+        case HSS_ZOCRS: return launch_sf_zocrs(a, mids->n, variant, traces->storage, st);
+        default: return launch_sf_oct(a, mids->n, variant, traces->storage, st);
     }
 }
 

@@ -1,0 +1,236 @@
+// Differential evolution over the traveltime parameters, one population per (midpoint, t0),
+// every generation inside one launch.
+//
+// Reference behaviour (common/differential_evolution.cu:19-155 driven by
+// DifferentialEvolutionAlgorithm.cpp:21-81): start x = lo + U*(hi-lo); evaluate; then
+// for G-1 generations mutate per parameter from donors floor(U*(NP-1)) with F = 0.85 and
+// clamp, cross over with CR = 0.5 and a forced index l = floor(U*(D-1)), evaluate the
+// trials and keep a trial on strict '>' of its semblance; finally report the first best
+// individual (cmp|zocrs|oct/differential_evolution.cu selectBestIndividuals*).  The
+// reference draws from curand seeded with time(); here the uniforms come from
+// Philox4x32-10 with counter {s, stream, individual, gen<<4 | block} and key = seed, the
+// layout the CPU oracle uses, so runs are reproducible and checkable.
+//
+// MI355X mapping: one workgroup = (midpoint, 64 consecutive t0); lane l of every wave
+// owns population s = tile*64 + l; wave v handles individuals v, v+4, ...  The
+// populations live in a caller-provided workspace, double buffered (trials of generation
+// g read generation g's individuals only), lane-fastest so every access is a 256-byte
+// coalesced row; one workgroup barrier separates generations.
+#pragma once
+
+#include "eval.hpp"
+#include "grid_impl.hpp"
+#include "launch_util.hpp"
+
+namespace hss {
+
+constexpr int kDeWaves = 4;
+constexpr int kDeThreads = kDeWaves * kWave;
+constexpr float kDeF = 0.85f;   // F_FAC, common/include/gpu/interface.h:5
+constexpr float kDeCR = 0.5f;   // CR, common/include/gpu/interface.h:6
+
+struct DeArgs {
+    EvalCtx e;
+    const int *mbeg;
+    const int *mcnt;
+    const float *mm0;
+    int NP, G;
+    float lo[3], hi[3];
+    uint32_t key0, key1, stream_base;
+    int ntiles;
+    float *ws;        // [n*ntiles][2][NP][D+2][64]
+    float *result;    // [n][2+D][ns]
+    float *pop_x;     // [n][ns][NP][D] or NULL
+    float *pop_fx;    // [n][ns][NP][2] or NULL
+};
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// 24-bit uniform in (0, 1].
+__device__ __forceinline__ float u01(uint32_t x) {
+    return ((float)(x >> 8) + 1.0f) * 5.9604644775390625e-08f;
+}
+
+// The first nq uniforms of (s, stream, i, gen).
+template <int NQ>
+__device__ __forceinline__ void de_uniforms(const DeArgs &a, uint32_t s, uint32_t stream, int i,
+                                            int gen, float *u) {
+#pragma unroll
+    for (int b = 0; b * 4 < NQ; ++b) {
+        uint32_t r[4];
+        philox4x32_10(s, stream, (uint32_t)i, ((uint32_t)gen << 4) | (uint32_t)b, a.key0, a.key1, r);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (b * 4 + q < NQ) u[b * 4 + q] = u01(r[q]);
+    }
+}
+
+// Storage-form candidate of an individual in v space (cmp|zocrs|oct/differential_evolution.cu:
+// c = 4/v^2, the other parameters as they are).
+template <int MODEL>
+__device__ __forceinline__ Candidate de_candidate(const float *x) {
+    Candidate k{0.0f, 0.0f, 0.0f};
+    const float v = x[0];
+    k.c = 4.0f / (v * v);
+    if constexpr (MODEL == HSS_ZOCRS) {
+        k.p1 = x[1];
+        k.p2 = x[2];
+    } else if constexpr (MODEL == HSS_OCT) {
+        k.p1 = x[1];
+    }
+    return k;
+}
+
+template <int MODEL, int W, int VARIANT, int STORAGE>
+__global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
+    constexpr int D = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
+    constexpr int REC = D + 2;  // D parameters, S, stack
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int g = blockIdx.x / a.ntiles;
+    const int tile = blockIdx.x - g * a.ntiles;
+    const int s = tile * kWave + lane;
+    const int ns = a.e.ns;
+    const bool active = s < ns;
+    const float t0 = (float)s * a.e.dt;
+    const float t0sq = t0 * t0;
+    const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
+    const uint32_t stream = a.stream_base + (uint32_t)g;
+    const int NP = a.NP;
+    float *base = a.ws + (size_t)blockIdx.x * 2 * NP * REC * kWave;
+    auto rec = [&](int buf, int i, int f) -> float * {
+        return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
+    };
+
+    // generation 0: the start population (startPopulations) and its fitness
+    for (int i = wave; i < NP; i += kDeWaves) {
+        float u[4];
+        de_uniforms<D>(a, (uint32_t)s, stream, i, 0, u);
+        float x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
+        float S, stk;
+        unsigned rej = 0;
+        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(x), 0.0f,
+                                                    active, &S, &stk, &rej);
+#pragma unroll
+        for (int d = 0; d < D; ++d) *rec(0, i, d) = x[d];
+        *rec(0, i, D) = S;
+        *rec(0, i, D + 1) = stk;
+    }
+    __syncthreads();
+
+    int cur = 0;
+    for (int gen = 1; gen < a.G; ++gen) {
+        for (int i = wave; i < NP; i += kDeWaves) {
+            constexpr int NQ = 4 * D + 1;
+            float u[NQ];
+            de_uniforms<NQ>(a, (uint32_t)s, stream, i, gen, u);
+            float trial[D];
+            // mutatePopulations (common/differential_evolution.cu:43-89)
+            float mutant[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const unsigned r1 = (unsigned)(u[3 * d] * (float)(NP - 1));
+                const unsigned r2 = (unsigned)(u[3 * d + 1] * (float)(NP - 1));
+                const unsigned r3 = (unsigned)(u[3 * d + 2] * (float)(NP - 1));
+                float nv = *rec(cur, r1, d) + kDeF * (*rec(cur, r2, d) - *rec(cur, r3, d));
+                nv = fminf(nv, a.hi[d]);
+                nv = fmaxf(nv, a.lo[d]);
+                mutant[d] = nv;
+            }
+            // crossoverPopulations (common/differential_evolution.cu:91-122)
+            const unsigned l = (unsigned)(u[3 * D] * (float)(D - 1));
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const float r = u[3 * D + 1 + d];
+                trial[d] = (r > kDeCR && l != (unsigned)d) ? *rec(cur, i, d) : mutant[d];
+            }
+            float S, stk;
+            unsigned rej = 0;
+            evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(trial),
+                                                        0.0f, active, &S, &stk, &rej);
+            // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
+            const bool take = S > *rec(cur, i, D);
+#pragma unroll
+            for (int d = 0; d < D; ++d) *rec(cur ^ 1, i, d) = take ? trial[d] : *rec(cur, i, d);
+            *rec(cur ^ 1, i, D) = take ? S : *rec(cur, i, D);
+            *rec(cur ^ 1, i, D + 1) = take ? stk : *rec(cur, i, D + 1);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+
+    if (wave != 0 || !active) return;
+    // selectBestIndividuals*: first strict maximum of S over the population
+    float best = -1.0f, bstack = 0.0f;
+    int bi = -1;
+    for (int i = 0; i < NP; ++i) {
+        const float S = *rec(cur, i, D);
+        if (S > best) {
+            best = S;
+            bstack = *rec(cur, i, D + 1);
+            bi = i;
+        }
+    }
+    float *res = a.result + (size_t)g * (2 + D) * ns;
+    res[s] = best;
+    res[ns + s] = bi >= 0 ? bstack : 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) res[(2 + d) * ns + s] = bi >= 0 ? *rec(cur, bi, d) : 0.0f;
+    if (a.pop_x) {
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                a.pop_x[(((size_t)g * ns + s) * NP + i) * D + d] = *rec(cur, i, d);
+    }
+    if (a.pop_fx) {
+        for (int i = 0; i < NP; ++i) {
+            a.pop_fx[(((size_t)g * ns + s) * NP + i) * 2] = *rec(cur, i, D);
+            a.pop_fx[(((size_t)g * ns + s) * NP + i) * 2 + 1] = *rec(cur, i, D + 1);
+        }
+    }
+}
+
+template <int MODEL>
+int launch_de(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+    int st = HSS_OK;
+    const unsigned blocks = (unsigned)n_mid * (unsigned)a.ntiles;
+    if (blocks == 0) return HSS_OK;
+    int rc = dispatch_storage(storage, [&](auto S) {
+        st = dispatch_variant(variant, [&](auto V) {
+            dispatch_window(a.e.w, [&](auto Wc) {
+                hipLaunchKernelGGL((de_semblance_kernel<MODEL, decltype(Wc)::value,
+                                                        decltype(V)::value, decltype(S)::value>),
+                                   dim3(blocks), dim3(kDeThreads), 0, stream, a);
+            });
+        });
+    });
+    if (rc != HSS_OK) return rc;
+    if (st != HSS_OK) return st;
+    return hip_status(hipGetLastError());
+}
+
+int launch_de_cmp(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_de_zocrs(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_de_oct(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+
+}  // namespace hss

@@ -86,12 +86,14 @@ struct TraceGeom {
 //   CMP    cmp/linear_search.cu:60-62
 //   ZO-CRS zocrs/linear_search.cu:83-90
 //   OCT    oct/linear_search.cu:84-98
+// t0sq is t0*t0 rounded, hoisted out of the trace loop.
 template <int MODEL>
-__device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float m0, float h0,
-                                           float apm, const Candidate &k, float *t, bool *reject) {
+__device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float t0sq, float m0,
+                                           float h0, float apm, const Candidate &k, float *t,
+                                           bool *reject) {
     *reject = false;
     if constexpr (MODEL == HSS_CMP) {
-        *t = sqrtf(t0 * t0 + k.c * tg.h2);
+        *t = sqrtf(t0sq + k.c * tg.h2);
         return true;
     } else if constexpr (MODEL == HSS_ZOCRS) {
         float dm = tg.m - m0;

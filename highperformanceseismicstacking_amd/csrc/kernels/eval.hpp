@@ -1,0 +1,91 @@
+// One (t0, candidate) semblance evaluation by one lane, shared by the grid, differential
+// evolution and stretch-free kernels.
+//
+// The lane walks the midpoint's traces in order -- the order of the reference's
+// per-thread loop (cmp/linear_search.cu:55-76 and twins) -- and computes the traveltime,
+// t/dt, the sample index and the interpolation weight with the reference's IEEE
+// operations in both variants: one ulp of t/dt is up to 6e-5 of interpolation weight
+// at t/dt ~ 1000, so the index chain must round as the reference's does for the results
+// to agree within 1e-5.  HSS_EXACT then reproduces an IEEE FP32 evaluation of the
+// reference text bit for bit; HSS_FAST contracts the window sums (window.hpp).
+#pragma once
+
+#include "device_common.hpp"
+#include "window.hpp"
+
+namespace hss {
+
+struct EvalCtx {
+    const void *samples;
+    const float *scale;
+    const float *mid;
+    const float *hoff;
+    const float *hoff2;
+    const int *midx;  // NULL: traces beg .. beg+cnt-1
+    int beg, cnt;
+    int ns, tau, w;
+    float dt;
+    float m0, h0, apm;
+};
+
+template <int MODEL>
+__device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
+                                               const float *__restrict__ hoff,
+                                               const float *__restrict__ hoff2, int ti) {
+    TraceGeom g{0.0f, 0.0f, 0.0f};
+    if constexpr (MODEL == HSS_CMP) {
+        g.h2 = hoff2[ti];
+    } else if constexpr (MODEL == HSS_ZOCRS) {
+        g.m = mid[ti];
+        g.h2 = hoff2[ti];
+    } else {
+        g.m = mid[ti];
+        g.h = hoff[ti];
+    }
+    return g;
+}
+
+// SHIFT: stretch-free evaluation, t = t_n + n*dt (cmp/stretch_free.cu:54-56); `shift` is
+// that n*dt.  Returns S and stack through the pointers and adds the number of model
+// rejections (OCT notUsedCount) to *rejected.
+template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT>
+__device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
+                                         const Candidate &k, float shift, bool active,
+                                         float *S, float *stack, unsigned *rejected) {
+    using Reader = SampleReader<STORAGE>;
+    const bool scaled = (STORAGE != HSS_F32) && e.scale != nullptr;
+    Reader rd{(decltype(Reader::p))e.samples};
+    Window<W, VARIANT> acc;
+    acc.reset();
+    unsigned rej = 0;
+    for (int j = 0; j < e.cnt; ++j) {
+        const int ti = e.midx ? e.midx[e.beg + j] : e.beg + j;
+        const TraceGeom tg = load_geom<MODEL>(e.mid, e.hoff, e.hoff2, ti);
+        int first;
+        float f;
+        float t;
+        bool r;
+        bool ok = traveltime<MODEL>(tg, t0, t0sq, e.m0, e.h0, e.apm, k, &t, &r);
+        rej += r ? 1u : 0u;
+        if constexpr (SHIFT) t = t + shift;
+        ok = ok && window_index(t, e.dt, e.ns, e.tau, &first, &f);
+        if (ok && active) {
+            const float sc = scaled ? e.scale[ti] : 1.0f;
+            acc.add(rd, (size_t)ti * (size_t)e.ns + (size_t)first, f, sc, scaled, e.w);
+        }
+    }
+    acc.reduce(e.w, S, stack);
+    *rejected += rej;
+}
+
+// Context for midpoint g of a batch (fields that do not depend on g come from `base`).
+__device__ __forceinline__ EvalCtx midpoint_ctx(const EvalCtx &base, const int *mbeg,
+                                                const int *mcnt, const float *mm0, int g) {
+    EvalCtx e = base;
+    e.beg = mbeg[g];
+    e.cnt = mcnt[g];
+    e.m0 = mm0[g];
+    return e;
+}
+
+}  // namespace hss

@@ -15,8 +15,8 @@
 // (larger value, then lower index), which reproduces the reference's sequential scan.
 #pragma once
 
+#include "eval.hpp"
 #include "launch_util.hpp"
-#include "window.hpp"
 
 namespace hss {
 
@@ -24,21 +24,10 @@ constexpr int kGridWaves = 4;
 constexpr int kGridThreads = kGridWaves * kWave;
 
 struct GridArgs {
-    const void *samples;
-    const float *scale;
-    const float *mid;
-    const float *hoff;
-    const float *hoff2;
-    int ns;
-    float dt;
-    int tau;
-    int w;
+    EvalCtx e;  // per-midpoint fields are filled in by the kernel
     const int *mbeg;
     const int *mcnt;
     const float *mm0;
-    const int *midx;
-    float h0;
-    float apm;
     const float *params;
     int P;
     int ntiles;
@@ -64,44 +53,46 @@ __device__ __forceinline__ Candidate load_candidate(const float *__restrict__ pa
     return k;
 }
 
-template <int MODEL>
-__device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
-                                               const float *__restrict__ hoff,
-                                               const float *__restrict__ hoff2, int ti) {
-    TraceGeom g{0.0f, 0.0f, 0.0f};
-    if constexpr (MODEL == HSS_CMP) {
-        g.h2 = hoff2[ti];
-    } else if constexpr (MODEL == HSS_ZOCRS) {
-        g.m = mid[ti];
-        g.h2 = hoff2[ti];
-    } else {
-        g.m = mid[ti];
-        g.h = hoff[ti];
+// Merges the per-wave running bests of a workgroup (larger value first, then lower
+// index); returns the winner in wave 0's lanes.
+template <int NWAVES>
+__device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, int wave,
+                                            int lane, float *B, float *BS, int *BI) {
+    __shared__ float sh_best[NWAVES][kWave];
+    __shared__ float sh_stack[NWAVES][kWave];
+    __shared__ int sh_idx[NWAVES][kWave];
+    sh_best[wave][lane] = best;
+    sh_stack[wave][lane] = bstack;
+    sh_idx[wave][lane] = bidx;
+    __syncthreads();
+    *B = -1.0f;
+    *BS = 0.0f;
+    *BI = -1;
+#pragma unroll
+    for (int v = 0; v < NWAVES; ++v) {
+        const int iv = sh_idx[v][lane];
+        const float sv = sh_best[v][lane];
+        if (iv >= 0 && (sv > *B || (sv == *B && iv < *BI))) {
+            *B = sv;
+            *BS = sh_stack[v][lane];
+            *BI = iv;
+        }
     }
-    return g;
 }
 
 template <int MODEL, int W, int VARIANT, int STORAGE>
 __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a) {
-    using Reader = SampleReader<STORAGE>;
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
-    __shared__ float sh_best[kGridWaves][kWave];
-    __shared__ float sh_stack[kGridWaves][kWave];
-    __shared__ int sh_idx[kGridWaves][kWave];
-
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int g = blockIdx.x / a.ntiles;
     const int tile = blockIdx.x - g * a.ntiles;
     const int s = tile * kWave + lane;
-    const bool active = s < a.ns;
-    const float t0 = (float)s * a.dt;
-
-    const int beg = a.mbeg[g];
-    const int cnt = a.mcnt[g];
-    const float m0 = a.mm0[g];
-    const bool scaled = (STORAGE != HSS_F32) && a.scale != nullptr;
-    Reader rd{(decltype(Reader::p))a.samples};
+    const int ns = a.e.ns;
+    const bool active = s < ns;
+    const float t0 = (float)s * a.e.dt;
+    const float t0sq = t0 * t0;
+    const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
 
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
@@ -109,30 +100,14 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
 
     for (int q = wave; q < a.P; q += kGridWaves) {
         const Candidate k = load_candidate<MODEL>(a.params, q);
-        Window<W, VARIANT> acc;
-        acc.reset();
-        unsigned rej_q = 0;
-        for (int j = 0; j < cnt; ++j) {
-            const int ti = a.midx ? a.midx[beg + j] : beg + j;
-            const TraceGeom tg = load_geom<MODEL>(a.mid, a.hoff, a.hoff2, ti);
-            float t;
-            bool rej;
-            bool ok = traveltime<MODEL>(tg, t0, m0, a.h0, a.apm, k, &t, &rej);
-            rej_q += rej ? 1u : 0u;
-            int first;
-            float f;
-            ok = ok && window_index(t, a.dt, a.ns, a.tau, &first, &f);
-            if (ok && active) {
-                const float sc = scaled ? a.scale[ti] : 1.0f;
-                acc.add(rd, (size_t)ti * (size_t)a.ns + (size_t)first, f, sc, scaled, a.w);
-            }
-        }
         float S, stk;
-        acc.reduce(a.w, &S, &stk);
+        unsigned rej_q = 0;
+        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, k, 0.0f, active, &S, &stk,
+                                                    &rej_q);
         if (active) {
             rejected += rej_q;
-            if (a.sembl) a.sembl[((size_t)g * a.ns + s) * a.P + q] = S;
-            if (a.stack) a.stack[((size_t)g * a.ns + s) * a.P + q] = stk;
+            if (a.sembl) a.sembl[((size_t)g * ns + s) * a.P + q] = S;
+            if (a.stack) a.stack[((size_t)g * ns + s) * a.P + q] = stk;
             if (S > best) {
                 best = S;
                 bstack = stk;
@@ -146,35 +121,21 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
         if (lane == 0 && tot) atomicAdd(a.not_used + g, tot);
     }
 
-    sh_best[wave][lane] = best;
-    sh_stack[wave][lane] = bstack;
-    sh_idx[wave][lane] = bidx;
-    __syncthreads();
+    float B, BS;
+    int BI;
+    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, &B, &BS, &BI);
     if (wave != 0 || !active) return;
-
-    float B = -1.0f, BS = 0.0f;
-    int BI = -1;
-#pragma unroll
-    for (int v = 0; v < kGridWaves; ++v) {
-        const int iv = sh_idx[v][lane];
-        const float sv = sh_best[v][lane];
-        if (iv >= 0 && (sv > B || (sv == B && iv < BI))) {
-            B = sv;
-            BS = sh_stack[v][lane];
-            BI = iv;
-        }
-    }
-    float *res = a.result + (size_t)g * (2 + NP) * a.ns;
+    float *res = a.result + (size_t)g * (2 + NP) * ns;
     res[s] = B;
-    res[a.ns + s] = BI >= 0 ? BS : 0.0f;
+    res[ns + s] = BI >= 0 ? BS : 0.0f;
     if (BI >= 0) {
         const Candidate k = load_candidate<MODEL>(a.params, BI);
-        res[2 * a.ns + s] = 2 / sqrtf(k.c);
-        if constexpr (NP >= 2) res[3 * a.ns + s] = k.p1;
-        if constexpr (NP >= 3) res[4 * a.ns + s] = k.p2;
+        res[2 * ns + s] = 2 / sqrtf(k.c);
+        if constexpr (NP >= 2) res[3 * ns + s] = k.p1;
+        if constexpr (NP >= 3) res[4 * ns + s] = k.p2;
     } else {
 #pragma unroll
-        for (int d = 0; d < NP; ++d) res[(2 + d) * a.ns + s] = 0.0f;
+        for (int d = 0; d < NP; ++d) res[(2 + d) * ns + s] = 0.0f;
     }
 }
 
@@ -186,7 +147,7 @@ int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, hipStrea
     if (blocks == 0) return HSS_OK;
     int rc = dispatch_storage(storage, [&](auto S) {
         st = dispatch_variant(variant, [&](auto V) {
-            dispatch_window(a.w, [&](auto Wc) {
+            dispatch_window(a.e.w, [&](auto Wc) {
                 hipLaunchKernelGGL((grid_semblance_kernel<MODEL, decltype(Wc)::value,
                                                           decltype(V)::value, decltype(S)::value>),
                                    dim3(blocks), dim3(kGridThreads), 0, stream, a);

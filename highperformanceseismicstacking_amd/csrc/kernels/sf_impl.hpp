@@ -1,0 +1,126 @@
+// Stretch-free re-stack with the best-shift pick fused in.
+//
+// Reference behaviour (cmp/stretch_free.cu:4-66, zocrs/stretch_free.cu:4-85,
+// oct/stretch_free.cu:5-97, then common/stretch_free.cu:3-35): for every t0 sample s and
+// every shift n of nArray (-10..10, StretchFreeAlgorithm.cpp:77-82) take the parameters
+// picked earlier at sample s_n = s - n (velocity as v, c_n = 4/v^2), evaluate the
+// traveltime at t0_n = s_n*dt and shift it by n*dt; a shift whose s_n falls outside the
+// trace gives S = stack = 0.  The first strict maximum over the shifts wins and the
+// result is (S, stack, n).
+//
+// MI355X mapping: as the grid kernel -- one workgroup per (midpoint, 64 t0), lane = t0,
+// wave v takes shifts v, v+4, ...; here the candidate differs per lane (it comes from
+// sample s_n), and the four per-wave bests merge through LDS.
+#pragma once
+
+#include "eval.hpp"
+#include "grid_impl.hpp"
+#include "launch_util.hpp"
+
+namespace hss {
+
+constexpr int kSfMaxShifts = 64;
+
+struct SfArgs {
+    EvalCtx e;
+    const int *mbeg;
+    const int *mcnt;
+    const float *mm0;
+    const float *vparams;  // [n][np][ns]: velocity, then A,B or slope, per sample
+    float nvals[kSfMaxShifts];
+    int nN;
+    int ntiles;
+    float *result;  // [n][3][ns]
+    float *sembl;   // [n][ns][nN] or NULL
+    float *stack;
+    unsigned long long *not_used;
+};
+
+template <int MODEL, int W, int VARIANT, int STORAGE>
+__global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
+    constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int g = blockIdx.x / a.ntiles;
+    const int tile = blockIdx.x - g * a.ntiles;
+    const int s = tile * kWave + lane;
+    const int ns = a.e.ns;
+    const bool active = s < ns;
+    const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
+    const float *vp = a.vparams + (size_t)g * NP * ns;
+
+    float best = -1.0f, bstack = 0.0f;
+    int bidx = -1;
+    unsigned long long rejected = 0;
+    for (int q = wave; q < a.nN; q += kGridWaves) {
+        const int n = (int)a.nvals[q];
+        const int sn = s - n;
+        const bool valid = active && sn >= 0 && sn < ns;
+        const int sc = valid ? sn : 0;
+        Candidate k{0.0f, 0.0f, 0.0f};
+        const float v = vp[sc];
+        k.c = 4.0f / (v * v);
+        if constexpr (MODEL == HSS_ZOCRS) {
+            k.p1 = vp[ns + sc];
+            k.p2 = vp[2 * ns + sc];
+        } else if constexpr (MODEL == HSS_OCT) {
+            k.p1 = vp[ns + sc];
+        }
+        const float t0n = (float)sc * e.dt;
+        float S = 0.0f, stk = 0.0f;
+        unsigned rej = 0;
+        evaluate<MODEL, W, VARIANT, STORAGE, true>(e, t0n, t0n * t0n, k, (float)n * e.dt, valid,
+                                                   &S, &stk, &rej);
+        if (!valid) {
+            S = 0.0f;
+            stk = 0.0f;
+        }
+        if (active) {
+            if (valid) rejected += rej;
+            if (a.sembl) a.sembl[((size_t)g * ns + s) * a.nN + q] = S;
+            if (a.stack) a.stack[((size_t)g * ns + s) * a.nN + q] = stk;
+            if (S > best) {
+                best = S;
+                bstack = stk;
+                bidx = q;
+            }
+        }
+    }
+    if (MODEL == HSS_OCT && a.not_used) {
+        unsigned long long tot = wave_sum_u64(rejected);
+        if (lane == 0 && tot) atomicAdd(a.not_used + g, tot);
+    }
+    float B, BS;
+    int BI;
+    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, &B, &BS, &BI);
+    if (wave != 0 || !active) return;
+    float *res = a.result + (size_t)g * 3 * ns;
+    res[s] = B;
+    res[ns + s] = BI >= 0 ? BS : 0.0f;
+    res[2 * ns + s] = BI >= 0 ? a.nvals[BI] : 0.0f;
+}
+
+template <int MODEL>
+int launch_sf(const SfArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+    int st = HSS_OK;
+    const unsigned blocks = (unsigned)n_mid * (unsigned)a.ntiles;
+    if (blocks == 0) return HSS_OK;
+    int rc = dispatch_storage(storage, [&](auto S) {
+        st = dispatch_variant(variant, [&](auto V) {
+            dispatch_window(a.e.w, [&](auto Wc) {
+                hipLaunchKernelGGL((sf_semblance_kernel<MODEL, decltype(Wc)::value,
+                                                        decltype(V)::value, decltype(S)::value>),
+                                   dim3(blocks), dim3(kGridThreads), 0, stream, a);
+            });
+        });
+    });
+    if (rc != HSS_OK) return rc;
+    if (st != HSS_OK) return st;
+    return hip_status(hipGetLastError());
+}
+
+int launch_sf_cmp(const SfArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_sf_zocrs(const SfArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_sf_oct(const SfArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+
+}  // namespace hss

@@ -121,6 +121,75 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
                        float *d_result, float *d_sembl, float *d_stack,
                        unsigned long long *d_not_used, void *stream);
 
+/* Bytes of device workspace hss_de_semblance needs for n_mid midpoints of ns samples
+ * with populations of NP individuals (two generations of every population). */
+int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint64_t *bytes);
+
+/* Differential evolution of the traveltime parameters for a batch of midpoints: for every
+ * (midpoint, t0) an independent population of NP individuals in v space, bounds
+ * lower/upper[np] (host arrays: velocity, then A,B or slope), G generations in total
+ * (the start population counts as the first), F = 0.85, CR = 0.5, selection on strict '>'
+ * of the semblance.  Uniforms come from Philox4x32-10 with key `seed`; midpoint g uses
+ * stream stream_base + g, so a batch and per-midpoint calls with matching stream ids
+ * give identical results.
+ *   d_result  [n][(2+np)][ns]: S, stack, then the best individual's parameters (v space);
+ *   d_pop_x   [n][ns][NP][np], d_pop_fx [n][ns][NP][2]: the final populations (NULL to skip).
+ * Replaces setupRandomSeed/startPopulations/mutatePopulations/crossoverPopulations/
+ * nextGeneration/computeSemblancesFor<Model>/selectBestIndividualsFor<Model>, launched one
+ * by one with a device sync each by CudaDifferentialEvolutionAlgorithm
+ * (cuda/src/semblance/algorithm/CudaDifferentialEvolutionAlgorithm.cu:35-353) under the
+ * generation loop of DifferentialEvolutionAlgorithm::computeSemblanceAndParametersForMidpoint
+ * (common/src/semblance/algorithm/DifferentialEvolutionAlgorithm.cpp:21-81). */
+int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int model,
+                     const float *lower, const float *upper, int32_t NP, int32_t G,
+                     uint64_t seed, uint32_t stream_base, int variant,
+                     void *d_workspace, uint64_t workspace_bytes,
+                     float *d_result, float *d_pop_x, float *d_pop_fx, void *stream);
+
+/* Stretch-free re-stack for a batch of midpoints.  d_vparams [n][np][ns] holds, per
+ * sample, the parameters picked by an earlier search (velocity in v space, then A,B or
+ * slope: the per-CDP traces of the --stack-datafiles files); nvals[nN] (host, nN <= 64)
+ * the sample shifts (the reference uses -10..10).
+ *   d_result [n][3][ns]: S, stack, n;  d_sembl, d_stack [n][ns][nN] (NULL to skip);
+ *   d_not_used [n] OCT rejections, accumulated (NULL to skip).
+ * Replaces computeSemblancesFor<Model> (stretch_free) + selectBestSemblances launched by
+ * CudaStretchFreeAlgorithm::computeSemblanceAtGpuForMidpoint
+ * (cuda/src/semblance/algorithm/CudaStretchFreeAlgorithm.cu:28-126). */
+int hss_stretch_free(const hss_traces *traces, const hss_midpoints *mids, int model,
+                     const float *d_vparams, const float *nvals, int32_t nN, int variant,
+                     float *d_result, float *d_sembl, float *d_stack,
+                     unsigned long long *d_not_used, void *stream);
+
+/* OCT trace selection: which candidates are tried at every t0. */
+enum hss_oct_mode {
+    HSS_OCT_PAIRS = 0,       /* d_pairs[npairs][2] {c, slope}, the same for every t0 (grid) */
+    HSS_OCT_PER_SAMPLE = 1,  /* d_vparams[n][2][ns] {velocity, slope} of each t0 (stretch-free) */
+    HSS_OCT_DE_START = 2     /* the start population of hss_de_semblance with the same
+                                NP, lower, upper, seed and stream_base */
+};
+
+typedef struct hss_oct_candidates {
+    int32_t mode;
+    const float *d_pairs;
+    int32_t npairs;
+    const float *d_vparams;
+    int32_t NP;
+    float lower[2];
+    float upper[2];
+    uint64_t seed;
+    uint32_t stream_base;
+} hss_oct_candidates;
+
+/* Trace selection masks over the whole line for n midpoints: d_mask[n][ntr] = 1 when the
+ * trace is used for that midpoint.  CMP: m == m0; ZO-CRS: |m0 - m| <= apm; OCT: some t0
+ * and candidate give a displaced midpoint within apm (`oct` says which candidates).
+ * Replaces selectTracesFor<Model>[AndDifferentialEvolution] launched by the
+ * selectTracesToBeUsedForMidpoint overrides (CudaLinearSearchAlgorithm.cu:225-324,
+ * CudaDifferentialEvolutionAlgorithm.cu:112-183, CudaStretchFreeAlgorithm.cu:128-200). */
+int hss_select_traces(const hss_traces *traces, int model, int32_t n, const float *d_m0,
+                      float apm, float h0, const hss_oct_candidates *oct, uint8_t *d_mask,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,220 @@
+"""Differential evolution, stretch-free re-stacking and trace selection on the GPU (through
+the C ABI) against the CPU oracle.
+
+DE draws its uniforms from the same Philox4x32-10 stream as the oracle, so HSS_EXACT
+reproduces the oracle's whole trajectory bit for bit (final populations included); the
+fast variant is judged the way the reference's own DE is: final semblance per sample
+within 1e-3, and the reported semblance must be that of the reported parameters.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from highperformanceseismicstacking_amd import CMP, ZOCRS, OCT, F32, F16, EXACT, FAST, capi, synth
+from helpers import compare_close, oracle_gather, zocrs_bounds, zocrs_window
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def de_case(model):
+    if model == CMP:
+        line = synth.make_line(3, (20, 36), 2500.0, 301, 4000, seed=21, offset_min=50.0)
+        b, c, m0 = line.cmp_ranges()
+        return line, b, c, m0, [1500.0], [4500.0], 0.0, 0.0, synth.tau_index(0.02, 4000)
+    if model == ZOCRS:
+        line, cfg = synth.workload("c3")
+        line = line.subset(0, 24 * 20)
+        lo, hi = zocrs_bounds(cfg["lower"][0], cfg["upper"][0], cfg["ain"], cfg["v0"], cfg["bpctg"])
+        _, _, m0all = line.cmp_ranges()
+        picks = m0all[[2, 9]]
+        rng = [zocrs_window(line, m, 60.0) for m in picks]
+        return (line, np.array([r[0] for r in rng]), np.array([r[1] for r in rng]), picks, lo, hi,
+                0.0, 60.0, synth.tau_index(0.02, 4000))
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 24)
+    _, _, m0all = line.cmp_ranges()
+    picks = m0all[[5, 15]]
+    rng = [zocrs_window(line, m, 40.0) for m in picks]
+    return (line, np.array([r[0] for r in rng]), np.array([r[1] for r in rng]), picks,
+            cfg["lower"], cfg["upper"], 150.0, 40.0, synth.tau_index(0.02, 4000))
+
+
+@pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
+def test_de_exact_reproduces_oracle_trajectory(model):
+    import torch
+    line, b, c, m0, lo, hi, h0, apm, tau = de_case(model)
+    NP, G, seed, base = 8, 6, 0x1234_5678_9abc, 7
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
+    res, px, pfx = capi.de_semblance(dl, mp, model, lo, hi, NP, G, seed, stream_base=base,
+                                     variant=EXACT, population=True)
+    torch.cuda.synchronize()
+    res, px, pfx = res.cpu().numpy(), px.cpu().numpy(), pfx.cpu().numpy()
+    for g in range(len(b)):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0, apm=apm)
+        r, x, fx = O.de_f32(model, og, lo, hi, NP, G, seed, stream_id=base + g, population=True)
+        assert np.array_equal(_bits(px[g]), _bits(x)), g
+        assert np.array_equal(_bits(pfx[g]), _bits(fx)), g
+        assert np.array_equal(_bits(res[g]), _bits(r)), g
+
+
+def test_de_batch_equals_single_midpoints():
+    import torch
+    line, b, c, m0, lo, hi, h0, apm, tau = de_case(CMP)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    res, _, _ = capi.de_semblance(dl, capi.DeviceMidpoints(b, c, m0), CMP, lo, hi, 12, 5, 42,
+                                  stream_base=100)
+    for g in range(len(b)):
+        r1, _, _ = capi.de_semblance(dl, capi.DeviceMidpoints(b[g:g + 1], c[g:g + 1], m0[g:g + 1]),
+                                     CMP, lo, hi, 12, 5, 42, stream_base=100 + g)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(res[g].cpu().numpy()), _bits(r1[0].cpu().numpy()))
+
+
+@pytest.mark.parametrize("model", [CMP, ZOCRS])
+def test_de_fast_statistics_and_consistency(model):
+    import torch
+    line, b, c, m0, lo, hi, h0, apm, tau = de_case(model)
+    NP, G, seed = 16, 12, 5
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
+    res, _, _ = capi.de_semblance(dl, mp, model, lo, hi, NP, G, seed, variant=FAST)
+    torch.cuda.synchronize()
+    res = res.cpu().numpy()
+    for g in range(len(b)):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), apm=apm)
+        r = O.de_f32(model, og, lo, hi, NP, G, seed, stream_id=g)
+        close = np.abs(res[g, 0] - r[0]) <= 1e-3
+        assert close.mean() >= 0.97, close.mean()
+        # the reported semblance is the semblance of the reported parameters
+        for s in range(0, line.ns, 13):
+            v = f32(res[g, 2, s])
+            prm = [f32(4.0) / (v * v)] + [res[g, 3 + d, s] for d in range(len(lo) - 1)]
+            S, T, _, _ = O.eval_f32(model, og, s, prm)
+            assert abs(S - res[g, 0, s]) <= 1e-5 * abs(S) + 1e-6
+
+
+def sf_case(model):
+    if model == CMP:
+        line = synth.make_line(3, (20, 36), 2500.0, 301, 4000, seed=31, offset_min=50.0)
+        b, c, m0 = line.cmp_ranges()
+        apm, h0 = 0.0, 0.0
+    elif model == ZOCRS:
+        line, _ = synth.workload("c3")
+        line = line.subset(0, 24 * 20)
+        _, _, m0all = line.cmp_ranges()
+        m0 = m0all[[3, 11]]
+        rng = [zocrs_window(line, m, 50.0) for m in m0]
+        b, c = np.array([r[0] for r in rng]), np.array([r[1] for r in rng])
+        apm, h0 = 50.0, 0.0
+    else:
+        line, _ = synth.workload("c4")
+        line = line.subset(0, 16 * 24)
+        _, _, m0all = line.cmp_ranges()
+        m0 = m0all[[4, 12]]
+        rng = [zocrs_window(line, m, 40.0) for m in m0]
+        b, c = np.array([r[0] for r in rng]), np.array([r[1] for r in rng])
+        apm, h0 = 40.0, 120.0
+    rng = np.random.default_rng(model + 40)
+    np_ = {CMP: 1, ZOCRS: 3, OCT: 2}[model]
+    vp = np.empty((len(b), np_, line.ns), np.float32)
+    vp[:, 0] = rng.uniform(1600, 4000, (len(b), line.ns))
+    if model == ZOCRS:
+        vp[:, 1] = rng.uniform(-5e-4, 5e-4, (len(b), line.ns))
+        vp[:, 2] = rng.uniform(-5e-7, 5e-7, (len(b), line.ns))
+    elif model == OCT:
+        vp[:, 1] = rng.uniform(-3e-4, 3e-4, (len(b), line.ns))
+    return line, b, c, m0, vp, apm, h0
+
+
+@pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
+@pytest.mark.parametrize("variant", [EXACT, FAST])
+def test_stretch_free_against_oracle(model, variant):
+    import torch
+    line, b, c, m0, vp, apm, h0 = sf_case(model)
+    tau = synth.tau_index(0.02, line.dt_us)
+    nvals = np.arange(-10, 11, dtype=np.float32)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
+    res, S, T, _ = capi.stretch_free(dl, mp, model, torch.from_numpy(vp).cuda(), nvals,
+                                     variant=variant, full=True, not_used=(model == OCT))
+    torch.cuda.synchronize()
+    res, S, T = res.cpu().numpy(), S.cpu().numpy(), T.cpu().numpy()
+    for g in range(len(b)):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0, apm=apm)
+        r, rS, rT = O.stretch_free_f32(model, og, vp[g].ravel(), nvals, full=True)
+        if variant == EXACT:
+            assert np.array_equal(_bits(S[g]), _bits(rS)), g
+            assert np.array_equal(_bits(T[g]), _bits(rT)), g
+            assert np.array_equal(_bits(res[g]), _bits(r)), g
+        else:
+            okS, okT = compare_close(S[g], T[g], rS, rT)
+            assert okS.all() and okT.all(), g
+
+
+def test_selection_cmp_and_zocrs_masks():
+    import torch
+    line, _ = synth.workload("c3")
+    tau = synth.tau_index(0.02, line.dt_us)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    _, _, m0all = line.cmp_ranges()
+    picks = m0all[::37]
+    for model, apm in [(CMP, 0.0), (ZOCRS, 250.0)]:
+        mask = capi.select_traces(dl, model, picks, apm=apm)
+        torch.cuda.synchronize()
+        mask = mask.cpu().numpy()
+        for g, m in enumerate(picks):
+            ref = O.select_traces(model, line.mid, line.hoff, m, apm=apm)
+            assert np.array_equal(mask[g], ref), (model, g)
+
+
+def test_selection_oct_three_candidate_sources():
+    import torch
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 40)
+    ns = 120
+    line = synth.Line(line.samples[:, :ns].copy(), line.sx, line.gx, line.scalco, ns, line.dt_us,
+                      line.cdp)
+    tau = synth.tau_index(0.02, line.dt_us)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    _, _, m0all = line.cmp_ranges()
+    picks = m0all[[3, 20, 33]]
+    apm, h0 = 30.0, 100.0
+    # grid: the default_random_engine draws of the reference
+    pairs = O.oct_selection_samples(1500.0, 4500.0, -5e-4, 5e-4, 64)
+    mask = capi.select_traces(dl, OCT, picks, apm=apm, h0=h0, oct_mode=capi.OCT_PAIRS,
+                              pairs=torch.from_numpy(pairs).cuda()).cpu().numpy()
+    for g, m in enumerate(picks):
+        ref = O.select_traces(OCT, line.mid, line.hoff, m, apm=apm, h0=h0, ns=ns, dt=line.dt,
+                              pairs=pairs, npairs=64)
+        assert np.array_equal(mask[g], ref), g
+    # stretch-free: per-sample (v, slope)
+    rng = np.random.default_rng(3)
+    vp = np.stack([rng.uniform(1500, 4500, (len(picks), ns)),
+                   rng.uniform(-5e-4, 5e-4, (len(picks), ns))], axis=1).astype(np.float32)
+    mask = capi.select_traces(dl, OCT, picks, apm=apm, h0=h0, oct_mode=capi.OCT_PER_SAMPLE,
+                              vparams=torch.from_numpy(vp).cuda()).cpu().numpy()
+    for g, m in enumerate(picks):
+        v = vp[g, 0]
+        pr = np.stack([f32(4.0) / (v * v), vp[g, 1]], axis=1).astype(np.float32).ravel()
+        ref = O.select_traces(OCT, line.mid, line.hoff, m, apm=apm, h0=h0, ns=ns, dt=line.dt,
+                              pairs=pr, npairs=1, per_sample=True)
+        assert np.array_equal(mask[g], ref), g
+    # differential evolution: the start population of the same seed and streams
+    NP, seed, base = 6, 77, 3
+    lo, hi = [1500.0, -5e-4], [4500.0, 5e-4]
+    mask = capi.select_traces(dl, OCT, picks, apm=apm, h0=h0, oct_mode=capi.OCT_DE_START, NP=NP,
+                              lower=lo, upper=hi, seed=seed, stream_base=base).cpu().numpy()
+    for g, m in enumerate(picks):
+        og = O.Gather(line.samples[:1], line.mid[:1], line.hoff[:1], line.dt, tau, m0=m, h0=h0, apm=apm)
+        _, x, _ = O.de_f32(OCT, og, lo, hi, NP, 1, seed, stream_id=base + g, population=True)
+        pr = np.stack([f32(4.0) / (x[..., 0] * x[..., 0]), x[..., 1]], axis=-1).astype(np.float32).ravel()
+        ref = O.select_traces(OCT, line.mid, line.hoff, m, apm=apm, h0=h0, ns=ns, dt=line.dt,
+                              pairs=pr, npairs=NP, per_sample=True)
+        assert np.array_equal(mask[g], ref), g
+        assert 0 < ref.sum() < line.ntr
