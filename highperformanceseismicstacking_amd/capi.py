@@ -80,8 +80,8 @@ def lib():
         L.hss_build_parameter_array.argtypes = [ci, vp, vp, vp, vp, vp]
         L.hss_grid_semblance.argtypes = [tr, mp, ci, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_de_workspace_size.argtypes = [i32, i32, ci, i32, ctypes.POINTER(u64)]
-        L.hss_de_semblance.argtypes = [tr, mp, ci, vp, vp, i32, i32, u64, u32, ci, vp, u64,
-                                       vp, vp, vp, vp]
+        L.hss_de_semblance.argtypes = [tr, mp, ci, vp, vp, i32, i32, i32, i32, u64, u32, ci, vp,
+                                       u64, vp, vp, vp, vp, vp]
         L.hss_stretch_free.argtypes = [tr, mp, ci, vp, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_select_traces.argtypes = [tr, ci, i32, vp, f32, f32,
                                         ctypes.POINTER(hss_oct_candidates), vp, vp]
@@ -252,8 +252,10 @@ def de_workspace_bytes(n_mid, ns, model, NP):
 
 
 def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, variant=FAST,
-                 population=False, result=None, workspace=None, stream=None):
-    """Returns (result [n,(2+np),ns], pop_x [n,ns,NP,np] or None, pop_fx [n,ns,NP,2] or None)."""
+                 population=False, result=None, workspace=None, stream=None, gen_range=None,
+                 not_used=None):
+    """Returns (result [n,(2+np),ns], pop_x [n,ns,NP,np] or None, pop_fx [n,ns,NP,2] or None).
+    gen_range=(g0, g1) runs generations [g0, g1) only, resuming from `workspace`."""
     import torch
     check_ranges(line, mids)
     dev = line.samples.device
@@ -261,6 +263,7 @@ def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, va
     lo = (ctypes.c_float * 3)(*[float(x) for x in list(lower)[:np_]] + [0.0] * (3 - np_))
     hi = (ctypes.c_float * 3)(*[float(x) for x in list(upper)[:np_]] + [0.0] * (3 - np_))
     nbytes = de_workspace_bytes(mids.n, line.ns, model, NP)
+    g0, g1 = gen_range if gen_range is not None else (0, G)
     if workspace is None:
         workspace = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device=dev)
     if result is None:
@@ -271,9 +274,10 @@ def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, va
         pfx = torch.empty((mids.n, line.ns, NP, 2), dtype=torch.float32, device=dev)
     check(lib().hss_de_semblance(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
                                  ctypes.cast(lo, ctypes.c_void_p), ctypes.cast(hi, ctypes.c_void_p),
-                                 int(NP), int(G), ctypes.c_uint64(seed), ctypes.c_uint32(stream_base),
-                                 int(variant), _ptr(workspace), ctypes.c_uint64(workspace.numel() * 4),
-                                 _ptr(result), _ptr(px), _ptr(pfx), _stream_handle(stream)))
+                                 int(NP), int(G), int(g0), int(g1), ctypes.c_uint64(seed),
+                                 ctypes.c_uint32(stream_base), int(variant), _ptr(workspace),
+                                 ctypes.c_uint64(workspace.numel() * 4), _ptr(result), _ptr(px),
+                                 _ptr(pfx), _ptr(not_used), _stream_handle(stream)))
     return result, px, pfx
 
 

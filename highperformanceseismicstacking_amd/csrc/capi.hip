@@ -178,13 +178,15 @@ int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint
 
 int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int model,
                      const float *lower, const float *upper, int32_t NP, int32_t G,
-                     uint64_t seed, uint32_t stream_base, int variant, void *d_workspace,
-                     uint64_t workspace_bytes, float *d_result, float *d_pop_x, float *d_pop_fx,
+                     int32_t gen_begin, int32_t gen_end, uint64_t seed, uint32_t stream_base,
+                     int variant, void *d_workspace, uint64_t workspace_bytes, float *d_result,
+                     float *d_pop_x, float *d_pop_fx, unsigned long long *d_not_used,
                      void *stream) {
     int rc = check_batch(traces, mids, model, variant);
     if (rc != HSS_OK) return rc;
     if (!lower || !upper || !d_result || !d_workspace) return HSS_ERR_NULL_POINTER;
     if (NP < 1 || G < 1) return HSS_ERR_INVALID_SIZE;
+    if (gen_begin < 0 || gen_begin >= gen_end || gen_end > G) return HSS_ERR_INVALID_SIZE;
     uint64_t need = 0;
     rc = hss_de_workspace_size(mids->n, traces->ns, model, NP, &need);
     if (rc != HSS_OK) return rc;
@@ -197,6 +199,9 @@ int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int mo
     a.mm0 = mids->m0;
     a.NP = NP;
     a.G = G;
+    a.gen_begin = gen_begin;
+    a.gen_end = gen_end;
+    a.not_used = d_not_used;
     const int np = model_param_count(model);
     for (int d = 0; d < 3; ++d) {
         a.lo[d] = d < np ? lower[d] : 0.0f;

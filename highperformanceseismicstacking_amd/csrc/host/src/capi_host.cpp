@@ -1,0 +1,113 @@
+// C API of the host layer (include/hss_host.h).
+#include "hss_host.h"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "logger.hpp"
+#include "model.hpp"
+#include "parser.hpp"
+#include "traveltime.hpp"
+
+static thread_local std::string last_error;
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        f();
+        last_error.clear();
+        return 0;
+    } catch (const std::invalid_argument &e) {
+        last_error = e.what();
+        return -1;
+    } catch (const std::logic_error &e) {
+        last_error = e.what();
+        return -2;
+    } catch (const std::runtime_error &e) {
+        last_error = e.what();
+        return -3;
+    } catch (const std::exception &e) {
+        last_error = e.what();
+        return -4;
+    }
+}
+
+extern "C" {
+
+const char *hss_host_last_error(void) { return last_error.c_str(); }
+
+int hss_host_read_gather(const char *path, float aph, float apm, float tau, float azimuth_deg,
+                         const float *lowest, const float *highest, int32_t *ns, float *dt,
+                         int32_t *tau_index, int32_t *w, int32_t *ntr, int32_t *ncdp) {
+    return guarded([&] {
+        Logger::getInstance()->setVerbosity(LogLevel::ERROR);
+        Gather::resetInstance();
+        Gather *g = Gather::getInstance();
+        g->setAph(aph);
+        g->setApm(apm);
+        g->setTau(tau);
+        g->setAzimuthInDegree(azimuth_deg);
+        if (lowest) g->setLowestAllowedMidpoint(*lowest);
+        if (highest) g->setBiggestAllowedMidpoint(*highest);
+        g->readGatherFromFile(path);
+        *ns = static_cast<int32_t>(g->getSamplesPerTrace());
+        *dt = g->getSamplePeriodInSeconds();
+        *tau_index = g->getTauIndexDisplacement();
+        *w = static_cast<int32_t>(g->getWindowSize());
+        *ntr = static_cast<int32_t>(g->getTotalTracesCount());
+        *ncdp = static_cast<int32_t>(g->getTotalCdpsCount());
+    });
+}
+
+int hss_host_gather_arrays(float *mid, float *hoff, float *samples, float *cdp_m0, int32_t *cdp_traces) {
+    return guarded([&] {
+        const Gather *g = Gather::getInstance();
+        const unsigned ns = g->getSamplesPerTrace();
+        for (unsigned i = 0; i < g->getTotalTracesCount(); i++) {
+            const Trace &t = g->getTraceAtIndex(i);
+            if (mid) mid[i] = t.getMidpoint();
+            if (hoff) hoff[i] = t.getHalfoffset();
+            if (samples) std::memcpy(samples + static_cast<size_t>(i) * ns, t.getSamples().data(), ns * sizeof(float));
+        }
+        size_t k = 0;
+        for (const auto &kv : g->getCdps()) {
+            if (cdp_m0) cdp_m0[k] = kv.first;
+            if (cdp_traces) cdp_traces[k] = static_cast<int32_t>(kv.second.getTraceCount());
+            k++;
+        }
+    });
+}
+
+int hss_host_traveltime_bounds(int model, const float *lower, const float *upper, int32_t n,
+                               float v0, float ain, float bpctg, float *out_lower, float *out_upper) {
+    return guarded([&] {
+        std::vector<float> lo(lower, lower + n), hi(upper, upper + n);
+        TraveltimeBuilder *b = TraveltimeBuilder::getInstance();
+        std::unique_ptr<Traveltime> t;
+        if (model == CMP) t.reset(b->buildCommonMidPoint(lo, hi));
+        else if (model == ZOCRS) t.reset(b->buildZeroOffsetCommonReflectionSurface(lo, hi, v0, ain, bpctg));
+        else if (model == OCT) t.reset(b->buildOffsetContinuationTrajectory(lo, hi, 0.0f));
+        else throw std::invalid_argument("Invalid traveltime model");
+        for (unsigned p = 0; p < t->getNumberOfParameters(); p++) {
+            out_lower[p] = t->getLowerBoundForParameter(p);
+            out_upper[p] = t->getUpperBoundForParameter(p);
+        }
+    });
+}
+
+int hss_host_parse_command_line(int program, int argc, const char **argv) {
+    return guarded([&] {
+        std::unique_ptr<Parser> p;
+        if (program == 0) p.reset(new LinearSearchParser());
+        else if (program == 1) p.reset(new DifferentialEvolutionParser());
+        else if (program == 2) p.reset(new StretchFreeParser());
+        else throw std::invalid_argument("unknown program");
+        p->parseArguments(argc, argv);
+        std::unique_ptr<Traveltime> t(p->parseTraveltime());
+        Logger::getInstance()->setVerbosity(LogLevel::ERROR);
+    });
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@ struct DeArgs {
     const int *mcnt;
     const float *mm0;
     int NP, G;
+    int gen_begin, gen_end;  // generations run by this launch; the workspace carries the rest
     float lo[3], hi[3];
     uint32_t key0, key1, stream_base;
     int ntiles;
@@ -42,6 +43,7 @@ struct DeArgs {
     float *result;    // [n][2+D][ns]
     float *pop_x;     // [n][ns][NP][D] or NULL
     float *pop_fx;    // [n][ns][NP][2] or NULL
+    unsigned long long *not_used;  // [n] OCT rejections, accumulated, or NULL
 };
 
 // Philox4x32-10 (Salmon et al., SC'11).
@@ -120,8 +122,11 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
     };
 
+    unsigned long long rejected = 0;
+    // Generation g's population lives in buffer g & 1 of the workspace, so a run split
+    // over several launches ([gen_begin, gen_end) each) gives the same result as one.
     // generation 0: the start population (startPopulations) and its fitness
-    for (int i = wave; i < NP; i += kDeWaves) {
+    for (int i = wave; a.gen_begin == 0 && i < NP; i += kDeWaves) {
         float u[4];
         de_uniforms<D>(a, (uint32_t)s, stream, i, 0, u);
         float x[D];
@@ -131,6 +136,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         unsigned rej = 0;
         evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(x), 0.0f,
                                                     active, &S, &stk, &rej);
+        if (active) rejected += rej;
 #pragma unroll
         for (int d = 0; d < D; ++d) *rec(0, i, d) = x[d];
         *rec(0, i, D) = S;
@@ -138,8 +144,8 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
     }
     __syncthreads();
 
-    int cur = 0;
-    for (int gen = 1; gen < a.G; ++gen) {
+    for (int gen = a.gen_begin > 1 ? a.gen_begin : 1; gen < a.gen_end; ++gen) {
+        const int cur = (gen - 1) & 1;
         for (int i = wave; i < NP; i += kDeWaves) {
             constexpr int NQ = 4 * D + 1;
             float u[NQ];
@@ -168,6 +174,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
             unsigned rej = 0;
             evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(trial),
                                                         0.0f, active, &S, &stk, &rej);
+            if (active) rejected += rej;
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
             const bool take = S > *rec(cur, i, D);
 #pragma unroll
@@ -176,10 +183,14 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
             *rec(cur ^ 1, i, D + 1) = take ? stk : *rec(cur, i, D + 1);
         }
         __syncthreads();
-        cur ^= 1;
     }
 
-    if (wave != 0 || !active) return;
+    if (MODEL == HSS_OCT && a.not_used) {
+        unsigned long long tot = wave_sum_u64(rejected);
+        if (lane == 0 && tot) atomicAdd(a.not_used + g, tot);
+    }
+    if (a.gen_end < a.G || wave != 0 || !active) return;
+    const int cur = (a.G - 1) & 1;
     // selectBestIndividuals*: first strict maximum of S over the population
     float best = -1.0f, bstack = 0.0f;
     int bi = -1;

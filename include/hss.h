@@ -132,8 +132,14 @@ int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint
  * of the semblance.  Uniforms come from Philox4x32-10 with key `seed`; midpoint g uses
  * stream stream_base + g, so a batch and per-midpoint calls with matching stream ids
  * give identical results.
+ *   gen_begin, gen_end: the generations this call runs, 0 <= gen_begin < gen_end <= G
+ *             ([0, G) is the whole run).  The populations stay in d_workspace between
+ *             calls, so consecutive ranges give the same result as one call; the outputs
+ *             below are written by the call that reaches gen_end = G;
  *   d_result  [n][(2+np)][ns]: S, stack, then the best individual's parameters (v space);
- *   d_pop_x   [n][ns][NP][np], d_pop_fx [n][ns][NP][2]: the final populations (NULL to skip).
+ *   d_pop_x   [n][ns][NP][np], d_pop_fx [n][ns][NP][2]: the final populations (NULL to skip);
+ *   d_not_used [n] OCT: (s, individual, trace) triples the model rejected, summed over the
+ *             generations run (NULL to skip; accumulated, caller zeroes it).
  * Replaces setupRandomSeed/startPopulations/mutatePopulations/crossoverPopulations/
  * nextGeneration/computeSemblancesFor<Model>/selectBestIndividualsFor<Model>, launched one
  * by one with a device sync each by CudaDifferentialEvolutionAlgorithm
@@ -142,9 +148,10 @@ int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint
  * (common/src/semblance/algorithm/DifferentialEvolutionAlgorithm.cpp:21-81). */
 int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int model,
                      const float *lower, const float *upper, int32_t NP, int32_t G,
-                     uint64_t seed, uint32_t stream_base, int variant,
-                     void *d_workspace, uint64_t workspace_bytes,
-                     float *d_result, float *d_pop_x, float *d_pop_fx, void *stream);
+                     int32_t gen_begin, int32_t gen_end, uint64_t seed, uint32_t stream_base,
+                     int variant, void *d_workspace, uint64_t workspace_bytes,
+                     float *d_result, float *d_pop_x, float *d_pop_fx,
+                     unsigned long long *d_not_used, void *stream);
 
 /* Stretch-free re-stack for a batch of midpoints.  d_vparams [n][np][ns] holds, per
  * sample, the parameters picked by an earlier search (velocity in v space, then A,B or

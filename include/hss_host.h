@@ -1,0 +1,46 @@
+/*
+ * hss_host.h -- C API of libhss_host.so, the C++17 host layer (SU reader, traveltime
+ * models, command line) of the MI355X semblance backend, for bindings and tests that
+ * need the host logic without a GPU.  None of these calls touches a device.
+ *
+ * Every function returns 0 on success, -1 when the host layer raised
+ * std::invalid_argument, -2 for std::logic_error, -3 for std::runtime_error and -4 for
+ * any other exception; hss_host_last_error() gives the message.
+ */
+#ifndef HSS_HOST_H
+#define HSS_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *hss_host_last_error(void);
+
+/* Reads an SU file with Gather::readGatherFromFile semantics (common/src/model/Gather.cpp:
+ * 49-126): lowest/highest may be NULL for the defaults (FLT_MIN, FLT_MAX).  Reports the
+ * scalars; hss_host_gather_arrays copies the kept traces (sorted by midpoint, then half
+ * offset) and the CDP map (ascending midpoint, with each CDP's kept-trace count). */
+int hss_host_read_gather(const char *path, float aph, float apm, float tau, float azimuth_deg,
+                         const float *lowest, const float *highest, int32_t *ns, float *dt,
+                         int32_t *tau_index, int32_t *w, int32_t *ntr, int32_t *ncdp);
+int hss_host_gather_arrays(float *mid, float *hoff, float *samples, float *cdp_m0,
+                           int32_t *cdp_traces);
+
+/* Parameter bounds TraveltimeBuilder derives (common/src/traveltime/TraveltimeBuilder.cpp:
+ * 23-101): model 0 CMP, 1 ZO-CRS (only lower/upper[0] are read; v0, ain, bpctg used),
+ * 2 OCT; out_lower/out_upper receive np values. */
+int hss_host_traveltime_bounds(int model, const float *lower, const float *upper, int32_t n,
+                               float v0, float ain, float bpctg, float *out_lower,
+                               float *out_upper);
+
+/* Parses a command line of program 0 (single_host_linear_search), 1 (single_host_de) or
+ * 2 (single_host_stretch_free) and builds its traveltime, without reading data or
+ * opening a device. */
+int hss_host_parse_command_line(int program, int argc, const char **argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
