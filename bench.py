@@ -129,7 +129,10 @@ def main():
         nbytes = args.calibrate_copy_mb << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
+        # write 1 GiB elsewhere first, so no part of src is still held in a cache
+        flush = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
+        del flush
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst

@@ -77,6 +77,7 @@ def lib():
         L.hss_status_string.restype = ctypes.c_char_p
         L.hss_status_string.argtypes = [ci]
         L.hss_device_count.argtypes = [ctypes.POINTER(ci)]
+        L.hss_verify_index_ops.argtypes = [f32, ctypes.POINTER(u64), vp]
         L.hss_build_parameter_array.argtypes = [ci, vp, vp, vp, vp, vp]
         L.hss_grid_semblance.argtypes = [tr, mp, ci, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_de_workspace_size.argtypes = [i32, i32, ci, i32, ctypes.POINTER(u64)]
@@ -191,6 +192,14 @@ class DeviceMidpoints:
         self.struct = hss_midpoints(
             self.n, self.begin.data_ptr(), self.count.data_ptr(), self.m0.data_ptr(),
             self.index.data_ptr() if self.index is not None else 0, float(h0), float(apm))
+
+
+def verify_index_ops(dt, stream=None):
+    """Mismatch count of the exhaustive check of the kernels' short sqrt and t/dt
+    sequences for this dt on the current device (0 means the kernels use them)."""
+    n = ctypes.c_uint64(0)
+    check(lib().hss_verify_index_ops(ctypes.c_float(dt), ctypes.byref(n), _stream_handle(stream)))
+    return int(n.value)
 
 
 def build_parameter_array(model, lower, step, count, device="cuda", stream=None):

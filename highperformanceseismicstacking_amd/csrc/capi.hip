@@ -56,6 +56,8 @@ static EvalCtx base_ctx(const hss_traces *t, const hss_midpoints *m) {
     e.tau = t->tau;
     e.w = t->w;
     e.dt = t->dt;
+    e.inv_dt = 1.0f / t->dt;
+    e.ib = index_bounds(t->ns, t->tau);
     e.h0 = m->h0;
     e.apm = m->apm;
     return e;

@@ -107,8 +107,9 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
     constexpr int REC = D + 2;  // D parameters, S, stack
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int g = blockIdx.x / a.ntiles;
-    const int tile = blockIdx.x - g * a.ntiles;
+    const int blk = logical_block(blockIdx.x, gridDim.x);
+    const int g = blk / a.ntiles;
+    const int tile = blk - g * a.ntiles;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
     const uint32_t stream = a.stream_base + (uint32_t)g;
     const int NP = a.NP;
-    float *base = a.ws + (size_t)blockIdx.x * 2 * NP * REC * kWave;
+    float *base = a.ws + (size_t)blk * 2 * NP * REC * kWave;
     auto rec = [&](int buf, int i, int f) -> float * {
         return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
     };
@@ -228,7 +229,7 @@ int launch_de(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t 
     if (blocks == 0) return HSS_OK;
     int rc = dispatch_storage(storage, [&](auto S) {
         st = dispatch_variant(variant, [&](auto V) {
-            dispatch_window(a.e.w, [&](auto Wc) {
+            dispatch_window(a.e.w, fast_index_verified(a.e.dt, a.e.tau, stream), [&](auto Wc) {
                 hipLaunchKernelGGL((de_semblance_kernel<MODEL, decltype(Wc)::value,
                                                         decltype(V)::value, decltype(S)::value>),
                                    dim3(blocks), dim3(kDeThreads), 0, stream, a);

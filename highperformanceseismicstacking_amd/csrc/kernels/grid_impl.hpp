@@ -85,8 +85,9 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int g = blockIdx.x / a.ntiles;
-    const int tile = blockIdx.x - g * a.ntiles;
+    const int blk = logical_block(blockIdx.x, gridDim.x);
+    const int g = blk / a.ntiles;
+    const int tile = blk - g * a.ntiles;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;
@@ -147,7 +148,7 @@ int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, hipStrea
     if (blocks == 0) return HSS_OK;
     int rc = dispatch_storage(storage, [&](auto S) {
         st = dispatch_variant(variant, [&](auto V) {
-            dispatch_window(a.e.w, [&](auto Wc) {
+            dispatch_window(a.e.w, fast_index_verified(a.e.dt, a.e.tau, stream), [&](auto Wc) {
                 hipLaunchKernelGGL((grid_semblance_kernel<MODEL, decltype(Wc)::value,
                                                           decltype(V)::value, decltype(S)::value>),
                                    dim3(blocks), dim3(kGridThreads), 0, stream, a);

@@ -19,7 +19,7 @@ inline int model_param_count(int model) {
 
 inline int check_traces(const hss_traces *t) {
     if (!t || !t->samples || !t->mid || !t->hoff || !t->hoff2) return HSS_ERR_NULL_POINTER;
-    if (t->ntr < 0 || t->ns <= 0) return HSS_ERR_INVALID_SIZE;
+    if (t->ntr < 0 || t->ns <= 0 || t->ns > (1 << 24)) return HSS_ERR_INVALID_SIZE;
     if (t->tau < 0 || t->w != 2 * t->tau + 1 || t->w > 32) return HSS_ERR_INVALID_WINDOW;
     if (t->storage < HSS_F32 || t->storage > HSS_BF16) return HSS_ERR_INVALID_STORAGE;
     if (!(t->dt > 0.0f)) return HSS_ERR_INVALID_SIZE;
@@ -35,10 +35,19 @@ inline int check_midpoints(const hss_midpoints *m) {
 
 inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; }
 
+// True when sqrt_rn / div_rn give the library's sample index and weight for this dt and
+// tau on the current device: tau >= 1 and the exhaustive check of verify.hip found no
+// mismatch (run once per device and dt, waiting on `stream` the first time).
+bool fast_index_verified(float dt, int tau, hipStream_t stream);
+
 // Calls f(std::integral_constant<int, W>) for the compiled window sizes; any other w
-// (odd, <= 32) goes to the runtime-w instantiation W = 0.
+// (odd, <= 32), or fast = false, goes to the runtime-w instantiation W = 0.
 template <class F>
-inline void dispatch_window(int w, F &&f) {
+inline void dispatch_window(int w, bool fast, F &&f) {
+    if (!fast) {
+        f(std::integral_constant<int, 0>{});
+        return;
+    }
     switch (w) {
         case 5: f(std::integral_constant<int, 5>{}); break;
         case 7: f(std::integral_constant<int, 7>{}); break;

@@ -1,0 +1,88 @@
+// Exhaustive check of the short correctly-rounded sequences of device_common.hpp.
+//
+// sqrt_rn(x) must equal sqrtf(x) for every non-negative float (and +inf), and
+// div_rn(t, dt, RN(1/dt)) must equal t/dt for every float t wherever a window with
+// tau >= 1 could accept either value: 1 <= t/dt < 2e9 (a window needs k = (int)(t/dt) >= tau,
+// window_index()).  Below that range the two may differ -- for |t| < ~2^-103 the residual
+// of div_rn is not exact, and -0 comes out as +0 -- but both values are then rejected
+// alike, so the sample index and the weight never differ.  The launchers use the short
+// sequences only for tau >= 1.  One launch walks all 2^32 bit patterns; the result is
+// cached per (device, dt).
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "launch_util.hpp"
+#include "device_common.hpp"
+
+namespace hss {
+
+__device__ unsigned long long verify_mismatches;
+
+__global__ __launch_bounds__(256) void verify_index_ops_kernel(float dt, float inv, int check_sqrt) {
+    unsigned long long bad = 0;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned long long b = blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32); b += stride) {
+        const float x = __uint_as_float((unsigned)b);
+        if (x != x) continue;  // NaN inputs never reach either sequence
+        if (check_sqrt && x >= 0.0f) {
+            if (__float_as_uint(sqrt_rn(x)) != __float_as_uint(sqrtf(x))) bad++;
+        }
+        const float e = x / dt;
+        const float f = div_rn(x, dt, inv);
+        const bool ge = e >= 1.0f && e < 2.0e9f;
+        const bool gf = f >= 1.0f && f < 2.0e9f;
+        if (ge != gf || (ge && __float_as_uint(e) != __float_as_uint(f))) bad++;
+    }
+    if (bad) atomicAdd(&verify_mismatches, bad);
+}
+
+// True when the short sequences may replace sqrtf and t/dt for this (dt, tau) on the
+// current device.  The first call per (device, dt) runs the check and waits for it.
+bool fast_index_verified(float dt, int tau, hipStream_t stream) {
+    if (tau < 1) return false;
+    static std::mutex mu;
+    static std::map<std::pair<int, unsigned>, bool> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    unsigned bits;
+    __builtin_memcpy(&bits, &dt, 4);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, bits});
+    if (it != cache.end()) return it->second;
+    const float inv = 1.0f / dt;
+    unsigned long long zero = 0, bad = 1;
+    bool ok = hipMemcpyToSymbolAsync(HIP_SYMBOL(verify_mismatches), &zero, sizeof(zero), 0,
+                                     hipMemcpyHostToDevice, stream) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(verify_index_ops_kernel, dim3(8192), dim3(256), 0, stream, dt, inv, 1);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyFromSymbolAsync(&bad, HIP_SYMBOL(verify_mismatches), sizeof(bad), 0,
+                                      hipMemcpyDeviceToHost, stream) == hipSuccess &&
+             hipStreamSynchronize(stream) == hipSuccess;
+    }
+    ok = ok && bad == 0;
+    cache[{dev, bits}] = ok;
+    return ok;
+}
+
+}  // namespace hss
+
+// Mismatch count of the check for dt on the current device (tests and diagnostics).
+extern "C" int hss_verify_index_ops(float dt, unsigned long long *mismatches, void *stream) {
+    using namespace hss;
+    if (!mismatches) return HSS_ERR_NULL_POINTER;
+    if (!(dt > 0.0f)) return HSS_ERR_INVALID_SIZE;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned long long zero = 0;
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(verify_mismatches), &zero, sizeof(zero), 0,
+                                          hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(verify_index_ops_kernel, dim3(8192), dim3(256), 0, st, dt, 1.0f / dt, 1);
+    e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    e = hipMemcpyFromSymbolAsync(mismatches, HIP_SYMBOL(verify_mismatches), sizeof(*mismatches), 0,
+                                 hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return (int)e;
+    return hip_status(hipStreamSynchronize(st));
+}

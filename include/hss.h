@@ -16,6 +16,9 @@
  *     0 = success, >0 = a hipError_t value, <0 = an hss_status argument error;
  *   - the caller owns all memory; no entry point allocates device memory, except
  *     where a declaration says "allocates" (none in this version);
+ *   - the first semblance call on a device for a given dt runs a one-time check of the
+ *     short square-root and division sequences the kernels use (hss_verify_index_ops)
+ *     and waits for it; later calls are fully asynchronous;
  *   - floating point in/out is float32; trace samples may be stored as float32,
  *     float16 or bfloat16 (hss_traces.storage) with a per-trace float32 scale.
  *
@@ -95,6 +98,16 @@ const char *hss_status_string(int status);
 /* Number of HIP devices (CudaSingleHostRunner::getNumOfDevices,
  * cuda/src/execution/CudaSingleHostRunner.cu:18-29). */
 int hss_device_count(int *count);
+
+/* Diagnostics (no counterpart in the reference): the number of float inputs, over all
+ * 2^32 bit patterns, on which the kernels' short correctly-rounded sequences differ from
+ * the library's correctly rounded operations on the current device: sqrt for every
+ * non-negative input, and t/dt from RN(1/dt) with one residual correction wherever either
+ * quotient lies in [1, 2e9) -- the only quotients a window with tau >= 1 can accept.  The
+ * semblance entry points use the short sequences for a dt only when this is 0 and
+ * tau >= 1, and the library operations otherwise; results are the same bits either way.
+ * Synchronous on `stream`. */
+int hss_verify_index_ops(float dt, unsigned long long *mismatches, void *stream);
 
 /* Grid candidates in storage form: CMP {c}, ZO-CRS {c,A,B}, OCT {c,slope}, c = 4/v^2,
  * v-major (ZO-CRS: B fastest; OCT: slope fastest); value d of axis k is

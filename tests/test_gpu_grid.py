@@ -73,6 +73,31 @@ def test_cmp_grid_exact_bitwise_c1():
     assert np.array_equal(first_max_index(S[0]), rbest)
 
 
+@pytest.mark.parametrize("dt", [0.004, 0.002, 0.001, 0.0005, 0.00025])
+def test_short_index_sequences_verified(dt):
+    """The exhaustive device check of the short sqrt and t/dt sequences finds no mismatch
+    for the usual sample intervals, so the compiled window sizes run with them."""
+    assert capi.verify_index_ops(dt) == 0
+
+
+@pytest.mark.parametrize("tau", [0, 1, 2, 7])
+def test_cmp_grid_exact_bitwise_window_edges(tau):
+    """Small and odd windows (w = 1 and 3 take the runtime-w kernel, 5 and 15 compiled
+    ones) on a short trace, so many windows cross both ends of the trace: EXACT equals the
+    oracle bit for bit, including which traces each window keeps."""
+    line = synth.make_line(1, 24, 1000.0, 61, 4000, seed=7, offsets=np.arange(0, 2400, 100.0),
+                           m_first=1000.0)
+    step = grid_steps([1500.0], [4500.0], [64])
+    params = O.build_parameter_array(O.CMP, [1500.0], step, [64])
+    b, c, m0 = line.cmp_ranges()
+    res, S, T, _, _ = run_gpu(line, b, c, m0, CMP, params, tau, EXACT)
+    g = oracle_gather(line.samples, line, int(b[0]), int(c[0]), tau, float(m0[0]))
+    rres, rS, rT, rbest, _ = O.grid_f32(O.CMP, g, params)
+    assert np.array_equal(S[0].view(np.uint32), rS.view(np.uint32))
+    assert np.array_equal(T[0].view(np.uint32), rT.view(np.uint32))
+    assert np.array_equal(res[0].view(np.uint32), rres.view(np.uint32))
+
+
 def test_cmp_grid_fast_tolerance_c1():
     line, cfg, tau, params = c1_problem()
     b, c, m0 = line.cmp_ranges()
