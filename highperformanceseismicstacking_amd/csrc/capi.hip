@@ -52,6 +52,9 @@ static EvalCtx base_ctx(const hss_traces *t, const hss_midpoints *m) {
     e.hoff = t->hoff;
     e.hoff2 = t->hoff2;
     e.midx = m->index;
+    e.ntr = t->ntr;
+    e.nsamp_odd = (int)(((size_t)t->ntr * (size_t)t->ns) & 1);
+    e.left_cap = 2 + (kSeg + 2) / t->ns;
     e.ns = t->ns;
     e.tau = t->tau;
     e.w = t->w;

@@ -123,6 +123,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
     };
 
+    __shared__ float seg_lds[kDeWaves][kSeg];
     unsigned long long rejected = 0;
     // Generation g's population lives in buffer g & 1 of the workspace, so a run split
     // over several launches ([gen_begin, gen_end) each) gives the same result as one.
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         float S, stk;
         unsigned rej = 0;
         evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(x), 0.0f,
-                                                    active, &S, &stk, &rej);
+                                                    active, seg_lds[wave], &S, &stk, &rej);
         if (active) rejected += rej;
 #pragma unroll
         for (int d = 0; d < D; ++d) *rec(0, i, d) = x[d];
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
             float S, stk;
             unsigned rej = 0;
             evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(trial),
-                                                        0.0f, active, &S, &stk, &rej);
+                                                        0.0f, active, seg_lds[wave], &S, &stk,
+                                                        &rej);
             if (active) rejected += rej;
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
             const bool take = S > *rec(cur, i, D);

@@ -17,6 +17,11 @@ namespace hss {
 constexpr int kWave = 64;
 constexpr int kMaxWindow = 32;  // MAX_WINDOW_SIZE, common/include/gpu/interface.h:10
 
+// A float in LDS.  Pointers of this type compile to ds_read/ds_write; a plain float*
+// into a __shared__ array is a generic pointer, and code that merges it with a global
+// one falls back to flat loads, which go through the vector memory path.
+using LdsFloat = __attribute__((address_space(3))) float;
+
 enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
     kNoError = 0,
     kDivisionByZero = 1,

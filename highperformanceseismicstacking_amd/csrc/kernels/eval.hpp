@@ -22,6 +22,9 @@ struct EvalCtx {
     const float *hoff;
     const float *hoff2;
     const int *midx;  // NULL: traces beg .. beg+cnt-1
+    int ntr;          // traces in the whole array
+    int nsamp_odd;    // (ntr * ns) & 1
+    int left_cap;     // 2 + (kSeg + 2) / ns: traces past which a segment never reaches
     int beg, cnt;
     int ns, tau, w;
     float dt, inv_dt;  // inv_dt = RN(1/dt)
@@ -46,17 +49,68 @@ __device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
     return g;
 }
 
+// Wave segments.  For one trace, the 64 lanes of a wave (consecutive t0 samples, one
+// candidate or nearby ones) read windows that overlap almost entirely: with moveout
+// dt/dt0 = t0/t <= 1 the window starts of a wave span at most 64 samples.  Instead of
+// (w+1) samples per lane through the vector memory path, the wave reads the kSeg samples
+// from the lowest window start on with one 8-byte load per lane, decoded and scaled, into
+// its own LDS slot, and each lane reads its window from there.  A trace whose windows do
+// not all fit one segment takes a second pass from the lowest remaining start (same
+// values, same order of traces per lane).
+constexpr int kSeg = 128;
+
+// Loads samples [g0, g0 + kSeg) of the flat sample array into seg[0..kSeg); `lim` is the
+// largest even lane offset that keeps both samples of a lane inside the array (16-bit
+// storage: inside the array's last aligned word), at most 2*kWave.  Positions past the
+// end of the array are left as they are; no window reads them.  16-bit storage needs g0
+// even (4-byte aligned words).  Every lane of the wave must call it.
+template <int STORAGE>
+__device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int lim, float sc,
+                                             bool scaled, LdsFloat *seg, int lane) {
+    const unsigned o = min((unsigned)(2 * lane), (unsigned)lim);
+    float v0, v1;
+    if constexpr (STORAGE == HSS_F32) {
+        const float *p = (const float *)e.samples + g0;
+        v0 = p[o];
+        v1 = p[o + 1];
+    } else {
+        const uint32_t *p = (const uint32_t *)((const uint16_t *)e.samples + g0);
+        const uint32_t word = p[o >> 1];
+        if constexpr (STORAGE == HSS_F16) {
+            v0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(word & 0xffffu));
+            v1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(word >> 16));
+        } else {
+            v0 = __uint_as_float(word << 16);
+            v1 = __uint_as_float(word & 0xffff0000u);
+        }
+    }
+    if (scaled) {
+        v0 = v0 * sc;
+        v1 = v1 * sc;
+    }
+    seg[o] = v0;
+    seg[o + 1] = v1;
+}
+
 // SHIFT: stretch-free evaluation, t = t_n + n*dt (cmp/stretch_free.cu:54-56); `shift` is
 // that n*dt.  Returns S and stack through the pointers and adds the number of model
 // rejections (OCT notUsedCount) to *rejected.
 // The compiled window sizes (W > 0) compute sqrt and t/dt with the verified short
 // sequences (sqrt_rn, div_rn); the runtime-w instantiation (W == 0) uses the library
 // operations and is also what the launchers pick when the check failed for this dt.
-template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT>
+// SEG (W > 0 only): read windows through the wave's LDS segment `seg` (kSeg floats,
+// private to the wave).  Every lane of the wave must call evaluate().  The segment pays
+// for 16-bit storage, whose per-lane path would read each sample with its own 2-byte
+// load; FP32 windows come in three 16-byte loads per lane, and there the segment's extra
+// instructions cost more than the vector memory path it relieves.
+template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32)>
 __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
                                          const Candidate &k, float shift, bool active,
-                                         float *S, float *stack, unsigned *rejected) {
+                                         float *seg_generic, float *S, float *stack,
+                                         unsigned *rejected) {
+    LdsFloat *seg = (LdsFloat *)seg_generic;
     using Reader = SampleReader<STORAGE>;
+    constexpr bool USE_SEG = SEG && W > 0;
     const bool scaled = (STORAGE != HSS_F32) && e.scale != nullptr;
     Reader rd{(decltype(Reader::p))e.samples};
     Window<W, VARIANT> acc;
@@ -80,8 +134,10 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
             tg.m = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.m), jj));
             tg.h = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h), jj));
             tg.h2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h2), jj));
-            int first;
-            float f;
+            const float sc =
+                scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
+            int first = 0;
+            float f = 0.0f;
             float t;
             bool r;
             constexpr bool FASTIDX = W != 0;
@@ -89,9 +145,39 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
             rej += r ? 1u : 0u;
             if constexpr (SHIFT) t = t + shift;
             ok = ok && window_index<FASTIDX>(t, e.dt, e.inv_dt, e.ib, e.tau, &first, &f);
-            if (ok && active) {
-                const float sc = scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
-                acc.add(rd, (size_t)ti * (size_t)e.ns + (size_t)first, f, sc, scaled, e.w);
+            const bool use = ok && active;
+            if constexpr (USE_SEG) {
+                // Segment passes: each pass starts at the lowest window start among the
+                // outermost pending lanes and serves every pending lane whose window lies
+                // inside the segment (normally all of them, so one pass).  The lane that
+                // set the start always fits, so every pass makes progress.
+                bool todo = use;
+                // samples from the start of trace ti to the end of the array, capped so
+                // the 32-bit products below cannot overflow (only the cap's lower bound
+                // matters: past kSeg + 2 samples the clamp is inactive)
+                const int left = min(e.ntr - ti, e.left_cap);
+                const int avail = left * e.ns;
+                unsigned long long pending = __builtin_amdgcn_ballot_w64(todo);
+                while (pending) {
+                    const int lo = __builtin_ctzll(pending);
+                    const int hi = 63 - __builtin_clzll(pending);
+                    const int b = min(__builtin_amdgcn_readlane(first, lo),
+                                      __builtin_amdgcn_readlane(first, hi));
+                    const size_t gb = (size_t)ti * (size_t)e.ns + (size_t)b;
+                    const int par = STORAGE == HSS_F32 ? 0 : (int)(gb & 1);
+                    // largest even (16-bit) lane offset with both samples in the array
+                    const int last = STORAGE == HSS_F32 ? avail - 2 - b
+                                                        : avail - 2 + e.nsamp_odd - b + par;
+                    load_segment<STORAGE>(e, gb - (size_t)par, min(last, 2 * kWave), sc, scaled,
+                                          seg, lane);
+                    const int rel = first - b + par;
+                    const bool fits = (unsigned)rel <= (unsigned)(kSeg - 1 - W);
+                    if (todo && fits) acc.add_lds(seg + rel, f, e.w);
+                    todo = todo && !fits;
+                    pending = __builtin_amdgcn_ballot_w64(todo);
+                }
+            } else {
+                if (use) acc.add(rd, (size_t)ti * (size_t)e.ns + (size_t)first, f, sc, scaled, e.w);
             }
         }
     }

@@ -95,6 +95,7 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
     const float t0sq = t0 * t0;
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
 
+    __shared__ float seg_lds[kGridWaves][kSeg];
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
@@ -103,8 +104,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
         const Candidate k = load_candidate<MODEL>(a.params, q);
         float S, stk;
         unsigned rej_q = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, k, 0.0f, active, &S, &stk,
-                                                    &rej_q);
+        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, k, 0.0f, active, seg_lds[wave],
+                                                    &S, &stk, &rej_q);
         if (active) {
             rejected += rej_q;
             if (a.sembl) a.sembl[((size_t)g * ns + s) * a.P + q] = S;

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
     const float *vp = a.vparams + (size_t)g * NP * ns;
 
+    __shared__ float seg_lds[kGridWaves][kSeg];
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
         float S = 0.0f, stk = 0.0f;
         unsigned rej = 0;
         evaluate<MODEL, W, VARIANT, STORAGE, true>(e, t0n, t0n * t0n, k, (float)n * e.dt, valid,
-                                                   &S, &stk, &rej);
+                                                   seg_lds[wave], &S, &stk, &rej);
         if (!valid) {
             S = 0.0f;
             stk = 0.0f;

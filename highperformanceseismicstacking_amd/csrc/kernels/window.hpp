@@ -57,6 +57,28 @@ struct Window {
         used++;
     }
 
+    // Same as add() for a window already decoded (and scaled) into LDS: row[0..w].
+    __device__ __forceinline__ void add_lds(const LdsFloat *row, float f, int w) {
+        float y1 = row[0];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (W == 0 && j >= w) continue;
+            float y0 = y1;
+            y1 = row[j + 1];
+            if constexpr (VARIANT == HSS_EXACT) {
+                float u = (y1 - y0) * f + y0;
+                A[j] += u;
+                lin += u;
+                den += u * u;
+            } else {
+                float u = __builtin_fmaf(y1 - y0, f, y0);
+                A[j] += u;
+                den = __builtin_fmaf(u, u, den);
+            }
+        }
+        used++;
+    }
+
     // REDUCE_SEMBLANCE_STACK: S = sum_j A_j^2 / (used * den), stack = lin / (used * w);
     // both stay 0 when no trace was used.
     __device__ __forceinline__ void reduce(int w, float *S, float *stack) const {
