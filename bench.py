@@ -29,9 +29,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=["c2", "c5"])
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--variant", default="fast", choices=["fast", "exact", "kahan"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU work budget of the oracle baseline sample (0 disables it)")
+    ap.add_argument("--cpl", type=int, default=0, choices=[0, 1, 2],
+                    help="pin the grid parameter tile (candidates per lane); 0 = autotune")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step directly instead of replaying a captured hipGraph")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the plain per-gather baseline and the FP16-storage timing")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the timed steps, run one plain device-to-device copy of this "
                          "many MiB (a known-size reference for rocprofv3 FETCH/WRITE counters)")
@@ -76,6 +82,53 @@ def cpu_baseline(line, tau, params, begins, counts, m0s, seconds):
                       f"({evals} evals, {el:.1f} s), FP32 C oracle, OpenMP over t0"}
 
 
+def measure_extras(args, line, tau, model, params, G, begins, counts, m0s, mp, variant,
+                   launch_ms):
+    """Same line, same GPU: (1) the plain per-gather FP32 kernel (the reference's kernel
+    structure, one launch pair per midpoint), timed once over the whole line; (2) the
+    batched kernel on FP16-storage traces (per-trace power-of-two scale)."""
+    import torch
+    from highperformanceseismicstacking_amd import F16, capi
+    out = {}
+    n_evals = int(mp.n) * line.ns * G
+    if args.storage == "f32":
+        dl32 = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+        scratch = (torch.empty((line.ns, G), dtype=torch.float32, device="cuda"),
+                   torch.empty((line.ns, G), dtype=torch.float32, device="cuda"))
+        res = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
+        capi.grid_semblance_plain(dl32, model, params, G, begins[:2], counts[:2], m0s[:2],
+                                  result=res, scratch=scratch)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        capi.grid_semblance_plain(dl32, model, params, G, begins, counts, m0s, result=res,
+                                  scratch=scratch)
+        e1.record()
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1)
+        out["plain_fp32_per_gather_ms"] = plain_ms
+        out["plain_fp32_per_gather_evals_per_s"] = n_evals / (plain_ms * 1e-3)
+        out["speedup_vs_plain_fp32"] = plain_ms / launch_ms
+        del scratch, res, dl32
+    dl16 = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=F16)
+    res = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
+    capi.grid_semblance(dl16, mp, model, params, G, variant=variant, result=res)
+    torch.cuda.synchronize()
+    reps = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        capi.grid_semblance(dl16, mp, model, params, G, variant=variant, result=res)
+    e1.record()
+    torch.cuda.synchronize()
+    f16_ms = e0.elapsed_time(e1) / reps
+    out["f16_storage_launch_ms"] = f16_ms
+    out["f16_storage_evals_per_s"] = n_evals / (f16_ms * 1e-3)
+    if "plain_fp32_per_gather_ms" in out:
+        out["f16_speedup_vs_plain_fp32"] = out["plain_fp32_per_gather_ms"] / f16_ms
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -88,10 +141,10 @@ def main():
     import numpy as np
     import torch
 
-    from highperformanceseismicstacking_amd import CMP, EXACT, FAST, F32, F16, BF16, capi, synth
+    from highperformanceseismicstacking_amd import CMP, EXACT, FAST, KAHAN, F32, F16, BF16, capi, synth
 
     storage = {"f32": F32, "f16": F16, "bf16": BF16}[args.storage]
-    variant = {"fast": FAST, "exact": EXACT}[args.variant]
+    variant = {"fast": FAST, "exact": EXACT, "kahan": KAHAN}[args.variant]
     line, cfg = synth.workload(args.workload)
     tau = synth.tau_index(cfg["tau"], line.dt_us)
     w = 2 * tau + 1
@@ -105,25 +158,53 @@ def main():
     result = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
 
+    if args.cpl:
+        capi.grid_set_config(CMP, w, variant, storage, args.cpl)
+
     def one_step():
         capi.grid_semblance(dl, mp, CMP, params, G, variant=variant, result=result)
 
-    for _ in range(args.warmup):
-        one_step()
+    # warm-up outside any capture: the first launch for a sample interval runs the
+    # one-time check of the short sqrt / division sequences (a synchronous call)
+    one_step()
     torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    graph = None
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            one_step()
+        stream.synchronize()
+        if not args.no_graph:
+            # the step as one hipGraph (captured on `stream`; replays go to the same stream)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=stream):
+                one_step()
+            graph.replay()
+        stream.synchronize()
 
-    stream = torch.cuda.current_stream()
+    def run_step():
+        if graph is not None:
+            graph.replay()
+        else:
+            one_step()
+
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        one_step()
-        ends[i].record(stream)
+    with torch.cuda.stream(stream):
+        for i in range(args.steps):
+            starts[i].record(stream)
+            run_step()
+            ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     launch_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+
+    extras = {}
+    if not args.no_extras:
+        extras = measure_extras(args, line, tau, CMP, params, G, begins, counts, m0s, mp, variant,
+                                launch_ms)
 
     if args.calibrate_copy_mb > 0:
         nbytes = args.calibrate_copy_mb << 20
@@ -166,6 +247,9 @@ def main():
             "variant": args.variant,
             "evals_per_step": n_evals,
             "launch_ms": launch_ms,
+            "graph": graph is not None,
+            "parameter_tile": capi.grid_tuned_config(CMP, w, variant, storage),
+            **extras,
         },
         "roofline": {
             "bound": "hbm",

@@ -14,7 +14,7 @@ BIN_DIR = os.path.join(PKG_DIR, "bin")
 
 CMP, ZOCRS, OCT = 0, 1, 2
 F32, F16, BF16 = 0, 1, 2
-EXACT, FAST = 0, 1
+EXACT, FAST, KAHAN = 0, 1, 2
 
 MODEL_NAMES = {"cmp": CMP, "zocrs": ZOCRS, "oct": OCT}
 MODEL_NPARAMS = {CMP: 1, ZOCRS: 3, OCT: 2}

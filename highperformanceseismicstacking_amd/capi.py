@@ -80,10 +80,16 @@ def lib():
         L.hss_verify_index_ops.argtypes = [f32, ctypes.POINTER(u64), vp]
         L.hss_build_parameter_array.argtypes = [ci, vp, vp, vp, vp, vp]
         L.hss_grid_semblance.argtypes = [tr, mp, ci, vp, i32, ci, vp, vp, vp, vp, vp]
+        L.hss_grid_semblance_plain.argtypes = [tr, ci, vp, i32, i32, vp, vp, vp, f32, f32, vp,
+                                               vp, vp, vp]
         L.hss_de_workspace_size.argtypes = [i32, i32, ci, i32, ctypes.POINTER(u64)]
         L.hss_de_semblance.argtypes = [tr, mp, ci, vp, vp, i32, i32, i32, i32, u64, u32, ci, vp,
                                        u64, vp, vp, vp, vp, vp]
         L.hss_stretch_free.argtypes = [tr, mp, ci, vp, vp, i32, ci, vp, vp, vp, vp, vp]
+        L.hss_semblance_gradient.argtypes = [tr, mp, ci, vp, vp, vp]
+        L.hss_grid_set_config.argtypes = [ci, ci, ci, ci, i32]
+        L.hss_grid_tuned_config.argtypes = [ci, ci, ci, ci, ctypes.POINTER(i32)]
+        L.hss_polish.argtypes = [tr, mp, ci, vp, vp, i32, vp, vp, vp]
         L.hss_select_traces.argtypes = [tr, ci, i32, vp, f32, f32,
                                         ctypes.POINTER(hss_oct_candidates), vp, vp]
         _lib = L
@@ -254,6 +260,30 @@ def grid_semblance(line, mids, model, params, P, variant=FAST, full=False,
     return result, sembl, stack, nu
 
 
+def grid_semblance_plain(line, model, params, P, begins, counts, m0s, h0=0.0, apm=0.0,
+                         result=None, scratch=None, stream=None):
+    """The plain per-gather baseline (one launch pair per midpoint, HSS_EXACT arithmetic).
+    begins/counts/m0s are host arrays; returns result [n][(2+np)][ns]."""
+    import numpy as np
+    import torch
+    np_ = {0: 1, 1: 3, 2: 2}[model]
+    b = np.ascontiguousarray(begins, dtype=np.int32)
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    m = np.ascontiguousarray(m0s, dtype=np.float32)
+    n = b.size
+    if result is None:
+        result = torch.empty((n, 2 + np_, line.ns), dtype=torch.float32, device="cuda")
+    if scratch is None:
+        scratch = (torch.empty((line.ns, P), dtype=torch.float32, device="cuda"),
+                   torch.empty((line.ns, P), dtype=torch.float32, device="cuda"))
+    check(lib().hss_grid_semblance_plain(
+        ctypes.byref(line.struct), model, _ptr(params), P, n,
+        b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p),
+        m.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(h0), ctypes.c_float(apm),
+        _ptr(scratch[0]), _ptr(scratch[1]), _ptr(result), _stream_handle(stream)))
+    return result
+
+
 def de_workspace_bytes(n_mid, ns, model, NP):
     out = ctypes.c_uint64(0)
     check(lib().hss_de_workspace_size(int(n_mid), int(ns), int(model), int(NP), ctypes.byref(out)))
@@ -346,3 +376,43 @@ def select_traces(line, model, m0, apm=0.0, h0=0.0, oct_mode=None, pairs=None, v
                                   ctypes.byref(octc) if octc is not None else None,
                                   _ptr(mask), _stream_handle(stream)))
     return mask
+
+
+def semblance_gradient(line, mids, model, points, out=None, stream=None):
+    """S, stack and dS/d(parameters) at the points in rows 2.. of `points`
+    [n][(2+np)][ns] (v space).  Returns out [n][(2+np)][ns]: S, stack, gradient rows."""
+    import torch
+    if out is None:
+        out = torch.empty_like(points)
+    check(lib().hss_semblance_gradient(ctypes.byref(line.struct), ctypes.byref(mids.struct),
+                                       int(model), _ptr(points), _ptr(out),
+                                       _stream_handle(stream)))
+    return out
+
+
+def polish(line, mids, model, lower, upper, steps, start, result=None, stream=None):
+    """Gauss-Newton polish of the points in rows 2.. of `start` [n][(2+np)][ns] within
+    [lower, upper] (v space).  Returns result [n][(2+np)][ns] (S, stack, parameters)."""
+    import numpy as np
+    import torch
+    lo = np.ascontiguousarray(lower, dtype=np.float32)
+    hi = np.ascontiguousarray(upper, dtype=np.float32)
+    if result is None:
+        result = torch.empty_like(start)
+    check(lib().hss_polish(ctypes.byref(line.struct), ctypes.byref(mids.struct), int(model),
+                           lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p),
+                           int(steps), _ptr(start), _ptr(result), _stream_handle(stream)))
+    return result
+
+
+def grid_set_config(model, w, variant, storage, cpl):
+    """Pins the grid launch shape (candidates per lane) for a key; 0 forgets it."""
+    check(lib().hss_grid_set_config(int(model), int(w), int(variant), int(storage), int(cpl)))
+
+
+def grid_tuned_config(model, w, variant, storage):
+    """Candidates per lane chosen for a key (0: not tuned yet)."""
+    c = ctypes.c_int32(0)
+    check(lib().hss_grid_tuned_config(int(model), int(w), int(variant), int(storage),
+                                      ctypes.byref(c)))
+    return int(c.value)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/hss.h"
+#include "kernels/adjoint.hpp"
 #include "kernels/de_impl.hpp"
 #include "kernels/grid_impl.hpp"
 #include "kernels/launch_util.hpp"
@@ -73,7 +74,8 @@ static int check_batch(const hss_traces *traces, const hss_midpoints *mids, int 
     rc = check_midpoints(mids);
     if (rc != HSS_OK) return rc;
     if (model_param_count(model) < 0) return HSS_ERR_INVALID_MODEL;
-    if (variant != HSS_EXACT && variant != HSS_FAST) return HSS_ERR_INVALID_VARIANT;
+    if (variant != HSS_EXACT && variant != HSS_FAST && variant != HSS_KAHAN)
+        return HSS_ERR_INVALID_VARIANT;
     return HSS_OK;
 }
 
@@ -164,11 +166,22 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
         return HSS_ERR_INVALID_SIZE;
     hipStream_t st = (hipStream_t)stream;
-    switch (model) {
-        case HSS_CMP: return launch_grid_cmp(a, mids->n, variant, traces->storage, st);
-        case HSS_ZOCRS: return launch_grid_zocrs(a, mids->n, variant, traces->storage, st);
-        default: return launch_grid_oct(a, mids->n, variant, traces->storage, st);
-    }
+    auto run = [&](const GridArgs &ga, int cpl) {
+        switch (model) {
+            case HSS_CMP: return launch_grid_cmp(ga, mids->n, variant, traces->storage, cpl, st);
+            case HSS_ZOCRS: return launch_grid_zocrs(ga, mids->n, variant, traces->storage, cpl, st);
+            default: return launch_grid_oct(ga, mids->n, variant, traces->storage, cpl, st);
+        }
+    };
+    // timing runs must not add to the accumulating rejection counts
+    GridArgs probe = a;
+    probe.not_used = nullptr;
+    int cpl = 1;
+    rc = tuned_grid_cpl(model, traces->w, variant, traces->storage,
+                        (unsigned)mids->n * (unsigned)a.ntiles, st,
+                        [&](int c) { return run(probe, c); }, &cpl);
+    if (rc != HSS_OK) return rc;
+    return run(a, cpl);
 }
 
 int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint64_t *bytes) {
@@ -260,6 +273,59 @@ int hss_stretch_free(const hss_traces *traces, const hss_midpoints *mids, int mo
         case HSS_ZOCRS: return launch_sf_zocrs(a, mids->n, variant, traces->storage, st);
         default: return launch_sf_oct(a, mids->n, variant, traces->storage, st);
     }
+}
+
+
+static int adjoint_common(const hss_traces *traces, const hss_midpoints *mids, int model,
+                          const float *d_points, float *d_out, const float *lower,
+                          const float *upper, int32_t steps, bool polish, void *stream) {
+    int rc = check_traces(traces);
+    if (rc != HSS_OK) return rc;
+    rc = check_midpoints(mids);
+    if (rc != HSS_OK) return rc;
+    const int np = model_param_count(model);
+    if (np < 0) return HSS_ERR_INVALID_MODEL;
+    if (!d_points || !d_out) return HSS_ERR_NULL_POINTER;
+    AdjointArgs a{};
+    a.e = base_ctx(traces, mids);
+    a.mbeg = mids->begin;
+    a.mcnt = mids->count;
+    a.mm0 = mids->m0;
+    a.ntiles = tiles_of(traces->ns);
+    a.points = d_points;
+    a.out = d_out;
+    if (polish) {
+        if (!lower || !upper) return HSS_ERR_NULL_POINTER;
+        if (steps < 0) return HSS_ERR_INVALID_SIZE;
+        for (int d = 0; d < np; ++d) {
+            if (!(lower[d] <= upper[d])) return HSS_ERR_INVALID_SIZE;
+            if (d == 0 && !(lower[0] > 0.0f)) return HSS_ERR_INVALID_SIZE;
+            a.lo[d] = lower[d];
+            a.hi[d] = upper[d];
+        }
+        a.steps = steps;
+    }
+    if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
+        return HSS_ERR_INVALID_SIZE;
+    if (mids->n == 0) return HSS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    switch (model) {
+        case HSS_CMP: return launch_adjoint_cmp(a, mids->n, traces->storage, polish, st);
+        case HSS_ZOCRS: return launch_adjoint_zocrs(a, mids->n, traces->storage, polish, st);
+        default: return launch_adjoint_oct(a, mids->n, traces->storage, polish, st);
+    }
+}
+
+int hss_semblance_gradient(const hss_traces *traces, const hss_midpoints *mids, int model,
+                           const float *d_points, float *d_out, void *stream) {
+    return adjoint_common(traces, mids, model, d_points, d_out, nullptr, nullptr, 0, false, stream);
+}
+
+int hss_polish(const hss_traces *traces, const hss_midpoints *mids, int model, const float *lower,
+               const float *upper, int32_t steps, const float *d_start, float *d_result,
+               void *stream) {
+    if (d_start == d_result) return HSS_ERR_INVALID_SIZE;
+    return adjoint_common(traces, mids, model, d_start, d_result, lower, upper, steps, true, stream);
 }
 
 }  // extern "C"

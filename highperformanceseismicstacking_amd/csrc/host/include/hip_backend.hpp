@@ -29,6 +29,7 @@ struct BackendOptions {
     unsigned int batchSize = 0;  // midpoints per launch in the runner; 0 = all of them
     bool seedGiven = false;
     uint64_t seed = 0;
+    unsigned int polishSteps = 0;  // Gauss-Newton steps applied to every winner (hss_polish)
     static BackendOptions &get();
 };
 
@@ -63,6 +64,11 @@ class MidpointBatch {
     bool indexed = false;
     DeviceBuffer dBeg, dCnt, dM0, dIndex;
 };
+
+// Replaces the search result in dRes ([n][(2+np)][ns] on the device) by its Gauss-Newton
+// polish inside the traveltime's parameter bounds when --polish-steps is set.
+void polish_results(const HipLine &line, const hss_midpoints &mids, const Traveltime &tt,
+                    DeviceBuffer &dRes, hipStream_t stream);
 
 // Trace selection on the sorted line: CMP m == m0 and ZO-CRS |m0 - m| <= apm are ranges
 // (found by binary search with the same float comparisons as cmp/common.cu:13 and

@@ -375,6 +375,7 @@ void HipLinearSearchAlgorithm::computeSemblanceAndParametersForMidpoints(
                                       HIP_DEV_PTR(deviceParameterArray), P,
                                       BackendOptions::get().variant, dRes.as<float>(), nullptr,
                                       nullptr, dNu.as<unsigned long long>(), stream()));
+        polish_results(line, mids, *traveltime, dRes, stream());
         kernelSec = timer.stopSeconds();
         dRes.download(res.data(), res.size() * sizeof(float), stream());
         dNu.download(nu.data(), nu.size() * sizeof(unsigned long long), stream());
@@ -422,6 +423,22 @@ void HipDifferentialEvolutionAlgorithm::setupRandomSeedArray() {
 uint32_t HipDifferentialEvolutionAlgorithm::streamOf(float m0) const {
     auto it = cdpOrdinal.find(m0);
     return it == cdpOrdinal.end() ? 0u : it->second;
+}
+
+void polish_results(const HipLine &line, const hss_midpoints &mids, const Traveltime &tt,
+                    DeviceBuffer &dRes, hipStream_t stream) {
+    const unsigned steps = BackendOptions::get().polishSteps;
+    if (steps == 0 || mids.n == 0) return;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (unsigned p = 0; p < tt.getNumberOfParameters() && p < 3; p++) {
+        lo[p] = tt.getLowerBoundForParameter(p);
+        hi[p] = tt.getUpperBoundForParameter(p);
+    }
+    DeviceBuffer dPol(dRes.size());
+    HSS_ASSERT(hss_polish(&line.traces(), &mids, tt.getModel(), lo, hi, static_cast<int32_t>(steps),
+                          dRes.as<float>(), dPol.as<float>(), stream));
+    HIP_ASSERT(hipMemcpyAsync(dRes.get(), dPol.get(), dRes.size(), hipMemcpyDeviceToDevice, stream));
+    HIP_ASSERT(hipStreamSynchronize(stream));
 }
 
 void HipDifferentialEvolutionAlgorithm::bounds(float lo[3], float hi[3]) const {
@@ -565,6 +582,7 @@ void HipDifferentialEvolutionAlgorithm::computeSemblanceAndParametersForMidpoint
                                     BackendOptions::get().variant, dWorkspace.get(),
                                     dWorkspace.size(), dRes.as<float>(), nullptr, nullptr,
                                     dNu.as<unsigned long long>(), stream()));
+        polish_results(line, mids, *traveltime, dRes, stream());
         const double kernelSec = timer.stopSeconds();
         std::vector<unsigned long long> nu(run.size());
         dRes.download(res.data(), res.size() * sizeof(float), stream());

@@ -129,10 +129,13 @@ Parser::Parser() {
     arguments.add("upper-bounds", K::FLOATS, "Upper bounds for traveltime parameters.", true);
     arguments.add("v0", K::FLOAT, "Velocity used to compute ZO-CRS A parameter.");
     arguments.add("verbose", K::UINT, "Verbosity level. Must be in the [0..4] integer range.");
-    arguments.add("variant", K::STRING, "Window arithmetic: exact (reference order) or fast (FMA).", false, "fast");
+    arguments.add("variant", K::STRING, "Window arithmetic: exact (reference order), fast (FMA) or kahan (FMA, compensated sums).", false, "fast");
     arguments.add("storage", K::STRING, "Trace sample storage on the device: f32, f16 or bf16.", false, "f32");
     arguments.add("batch-size", K::UINT, "Midpoints per kernel launch (0 = all).", false, "0");
     arguments.add("seed", K::STRING, "Differential-evolution seed (default: the clock).");
+    arguments.add("polish-steps", K::UINT,
+                  "Gauss-Newton steps on the semblance gradient applied to every winner (0 = none).",
+                  false, "0");
 }
 
 void Parser::parseArguments(int argc, const char *argv[]) {
@@ -141,14 +144,16 @@ void Parser::parseArguments(int argc, const char *argv[]) {
         Logger::getInstance()->setVerbosity(getLogVerbosity());
         BackendOptions &o = BackendOptions::get();
         const std::string variant = getString("variant");
-        if (variant != "exact" && variant != "fast") throw std::runtime_error("--variant must be exact or fast");
-        o.variant = variant == "exact" ? HSS_EXACT : HSS_FAST;
+        if (variant != "exact" && variant != "fast" && variant != "kahan")
+            throw std::runtime_error("--variant must be exact, fast or kahan");
+        o.variant = variant == "exact" ? HSS_EXACT : (variant == "kahan" ? HSS_KAHAN : HSS_FAST);
         const std::string storage = getString("storage");
         if (storage == "f32") o.storage = HSS_F32;
         else if (storage == "f16") o.storage = HSS_F16;
         else if (storage == "bf16") o.storage = HSS_BF16;
         else throw std::runtime_error("--storage must be f32, f16 or bf16");
         o.batchSize = getUInt("batch-size");
+        o.polishSteps = getUInt("polish-steps");
         o.seedGiven = has("seed");
         if (o.seedGiven) o.seed = std::stoull(getString("seed"));
     } catch (const std::exception &ex) {

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
     };
 
-    __shared__ float seg_lds[kDeWaves][kSeg];
+    __shared__ float seg_lds[kDeWaves][kSegFloats];
     unsigned long long rejected = 0;
     // Generation g's population lives in buffer g & 1 of the workspace, so a run split
     // over several launches ([gen_begin, gen_end) each) gives the same result as one.

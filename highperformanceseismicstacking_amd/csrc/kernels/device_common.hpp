@@ -22,6 +22,10 @@ constexpr int kMaxWindow = 32;  // MAX_WINDOW_SIZE, common/include/gpu/interface
 // one falls back to flat loads, which go through the vector memory path.
 using LdsFloat = __attribute__((address_space(3))) float;
 
+// Wave segment (eval.hpp): kSeg samples per wave in LDS.
+constexpr int kSeg = 128;
+constexpr int kSegFloats = kSeg;
+
 enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
     kNoError = 0,
     kDivisionByZero = 1,

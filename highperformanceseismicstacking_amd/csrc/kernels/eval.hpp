@@ -53,17 +53,26 @@ __device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
 // candidate or nearby ones) read windows that overlap almost entirely: with moveout
 // dt/dt0 = t0/t <= 1 the window starts of a wave span at most 64 samples.  Instead of
 // (w+1) samples per lane through the vector memory path, the wave reads the kSeg samples
-// from the lowest window start on with one 8-byte load per lane, decoded and scaled, into
-// its own LDS slot, and each lane reads its window from there.  A trace whose windows do
-// not all fit one segment takes a second pass from the lowest remaining start (same
-// values, same order of traces per lane).
-constexpr int kSeg = 128;
+// from the lowest window start on with one 4- or 8-byte load per lane, decodes and scales
+// them once into its own LDS slot, and each lane reads its window from there.  A trace
+// whose windows do not all fit one segment takes a second pass from the lowest remaining
+// start (same values, same order of traces per lane).  (Storing the sample differences
+// next to the samples saves one subtraction per window value but doubles the LDS bytes
+// each window reads, which costs more than it saves.)
+
+// Orders the wave's LDS accesses around cross-lane reads and writes: LDS instructions of
+// one wave execute in program order, so keeping the compiler from moving them suffices.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Loads samples [g0, g0 + kSeg) of the flat sample array into seg[0..kSeg); `lim` is the
-// largest even lane offset that keeps both samples of a lane inside the array (16-bit
-// storage: inside the array's last aligned word), at most 2*kWave.  Positions past the
-// end of the array are left as they are; no window reads them.  16-bit storage needs g0
-// even (4-byte aligned words).  Every lane of the wave must call it.
+// largest lane offset that keeps both samples of a lane inside the array (16-bit storage:
+// inside the array's last aligned word; then lim is even), at most 2*kWave.  Positions
+// past the end of the array are left as they are; no window reads them.  16-bit storage
+// needs g0 even (4-byte aligned words).  Every lane of the wave must call it.
 template <int STORAGE>
 __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int lim, float sc,
                                              bool scaled, LdsFloat *seg, int lane) {
@@ -84,12 +93,15 @@ __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int li
             v1 = __uint_as_float(word & 0xffff0000u);
         }
     }
-    if (scaled) {
+    if (STORAGE != HSS_F32 || scaled) {
+        // sc is 1 when the line has no scales (exact)
         v0 = v0 * sc;
         v1 = v1 * sc;
     }
+    wave_lds_order();  // the previous pass's windows have been read
     seg[o] = v0;
     seg[o + 1] = v1;
+    wave_lds_order();
 }
 
 // SHIFT: stretch-free evaluation, t = t_n + n*dt (cmp/stretch_free.cu:54-56); `shift` is
@@ -98,23 +110,30 @@ __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int li
 // The compiled window sizes (W > 0) compute sqrt and t/dt with the verified short
 // sequences (sqrt_rn, div_rn); the runtime-w instantiation (W == 0) uses the library
 // operations and is also what the launchers pick when the check failed for this dt.
+// CPL candidates per lane share the trace loop, the geometry broadcast and, with SEG,
+// the segment passes; `on[c]` false leaves candidate c out (its results are those of an
+// empty window sum).
 // SEG (W > 0 only): read windows through the wave's LDS segment `seg` (kSeg floats,
 // private to the wave).  Every lane of the wave must call evaluate().  The segment pays
 // for 16-bit storage, whose per-lane path would read each sample with its own 2-byte
 // load; FP32 windows come in three 16-byte loads per lane, and there the segment's extra
 // instructions cost more than the vector memory path it relieves.
-template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32)>
-__device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
-                                         const Candidate &k, float shift, bool active,
-                                         float *seg_generic, float *S, float *stack,
-                                         unsigned *rejected) {
+template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, int CPL,
+          bool SEG = (STORAGE != HSS_F32)>
+__device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0sq,
+                                           const Candidate (&k)[CPL], const bool (&on)[CPL],
+                                           float shift, bool active, float *seg_generic,
+                                           float (&S)[CPL], float (&stack)[CPL],
+                                           unsigned *rejected) {
     LdsFloat *seg = (LdsFloat *)seg_generic;
     using Reader = SampleReader<STORAGE>;
     constexpr bool USE_SEG = SEG && W > 0;
+    constexpr bool FASTIDX = W != 0;
     const bool scaled = (STORAGE != HSS_F32) && e.scale != nullptr;
     Reader rd{(decltype(Reader::p))e.samples};
-    Window<W, VARIANT> acc;
-    acc.reset();
+    Window<W, VARIANT> acc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].reset();
     unsigned rej = 0;
     const int lane = threadIdx.x & (kWave - 1);
     // Traces go by in chunks of 64: lane i loads trace j0+i's index, geometry and scale
@@ -136,33 +155,47 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
             tg.h2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h2), jj));
             const float sc =
                 scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
-            int first = 0;
-            float f = 0.0f;
-            float t;
-            bool r;
-            constexpr bool FASTIDX = W != 0;
-            bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k, &t, &r);
-            rej += r ? 1u : 0u;
-            if constexpr (SHIFT) t = t + shift;
-            ok = ok && window_index<FASTIDX>(t, e.dt, e.inv_dt, e.ib, e.tau, &first, &f);
-            const bool use = ok && active;
+            int first[CPL];
+            float f[CPL];
+            bool use[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                first[c] = 0;
+                f[c] = 0.0f;
+                float t;
+                bool r;
+                bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
+                rej += (r && on[c]) ? 1u : 0u;
+                if constexpr (SHIFT) t = t + shift;
+                ok = ok && window_index<FASTIDX>(t, e.dt, e.inv_dt, e.ib, e.tau, &first[c], &f[c]);
+                use[c] = ok && active && on[c];
+            }
             if constexpr (USE_SEG) {
                 // Segment passes: each pass starts at the lowest window start among the
-                // outermost pending lanes and serves every pending lane whose window lies
-                // inside the segment (normally all of them, so one pass).  The lane that
-                // set the start always fits, so every pass makes progress.
-                bool todo = use;
+                // outermost pending lanes and serves every pending window that lies inside
+                // the segment (normally all of them, so one pass for all CPL candidates).
+                // The window that set the start always fits, so every pass makes progress.
+                bool todo[CPL];
+                bool any = false;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    todo[c] = use[c];
+                    any = any || use[c];
+                }
                 // samples from the start of trace ti to the end of the array, capped so
                 // the 32-bit products below cannot overflow (only the cap's lower bound
                 // matters: past kSeg + 2 samples the clamp is inactive)
                 const int left = min(e.ntr - ti, e.left_cap);
                 const int avail = left * e.ns;
-                unsigned long long pending = __builtin_amdgcn_ballot_w64(todo);
+                unsigned long long pending = __builtin_amdgcn_ballot_w64(any);
                 while (pending) {
+                    int fmin = 0x7fffffff;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) fmin = todo[c] ? min(fmin, first[c]) : fmin;
                     const int lo = __builtin_ctzll(pending);
                     const int hi = 63 - __builtin_clzll(pending);
-                    const int b = min(__builtin_amdgcn_readlane(first, lo),
-                                      __builtin_amdgcn_readlane(first, hi));
+                    const int b = min(__builtin_amdgcn_readlane(fmin, lo),
+                                      __builtin_amdgcn_readlane(fmin, hi));
                     const size_t gb = (size_t)ti * (size_t)e.ns + (size_t)b;
                     const int par = STORAGE == HSS_F32 ? 0 : (int)(gb & 1);
                     // largest even (16-bit) lane offset with both samples in the array
@@ -170,19 +203,45 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
                                                         : avail - 2 + e.nsamp_odd - b + par;
                     load_segment<STORAGE>(e, gb - (size_t)par, min(last, 2 * kWave), sc, scaled,
                                           seg, lane);
-                    const int rel = first - b + par;
-                    const bool fits = (unsigned)rel <= (unsigned)(kSeg - 1 - W);
-                    if (todo && fits) acc.add_lds(seg + rel, f, e.w);
-                    todo = todo && !fits;
-                    pending = __builtin_amdgcn_ballot_w64(todo);
+                    bool more = false;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        const int rel = first[c] - b + par;
+                        const bool fits = (unsigned)rel <= (unsigned)(kSeg - 1 - W);
+                        if (todo[c] && fits) acc[c].add_lds(seg + rel, f[c], e.w);
+                        todo[c] = todo[c] && !fits;
+                        more = more || todo[c];
+                    }
+                    pending = __builtin_amdgcn_ballot_w64(more);
                 }
             } else {
-                if (use) acc.add(rd, (size_t)ti * (size_t)e.ns + (size_t)first, f, sc, scaled, e.w);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    if (use[c])
+                        acc[c].add(rd, (size_t)ti * (size_t)e.ns + (size_t)first[c], f[c], sc, scaled,
+                                   e.w);
+                }
             }
         }
     }
-    acc.reduce(e.w, S, stack);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].reduce(e.w, &S[c], &stack[c]);
     *rejected += rej;
+}
+
+// One candidate per lane.
+template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32)>
+__device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
+                                         const Candidate &k, float shift, bool active,
+                                         float *seg_generic, float *S, float *stack,
+                                         unsigned *rejected) {
+    const Candidate kk[1] = {k};
+    const bool on[1] = {true};
+    float s1[1], t1[1];
+    evaluate_n<MODEL, W, VARIANT, STORAGE, SHIFT, 1, SEG>(e, t0, t0sq, kk, on, shift, active,
+                                                          seg_generic, s1, t1, rejected);
+    *S = s1[0];
+    *stack = t1[0];
 }
 
 // Context for midpoint g of a batch (fields that do not depend on g come from `base`).

@@ -80,7 +80,10 @@ __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, 
     }
 }
 
-template <int MODEL, int W, int VARIANT, int STORAGE>
+// CPL: candidates per lane (the parameter tile): wave v takes the candidate groups
+// [q0, q0 + CPL) with q0 = CPL*v, CPL*(v + 4), ...; within a lane candidates are taken in
+// increasing order, so the running best is still the first strict maximum.
+template <int MODEL, int W, int VARIANT, int STORAGE, int CPL>
 __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a) {
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
@@ -95,25 +98,36 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
     const float t0sq = t0 * t0;
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
 
-    __shared__ float seg_lds[kGridWaves][kSeg];
+    __shared__ float seg_lds[kGridWaves][kSegFloats];
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
 
-    for (int q = wave; q < a.P; q += kGridWaves) {
-        const Candidate k = load_candidate<MODEL>(a.params, q);
-        float S, stk;
+    for (int q0 = wave * CPL; q0 < a.P; q0 += kGridWaves * CPL) {
+        Candidate k[CPL];
+        bool on[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            on[c] = q0 + c < a.P;
+            k[c] = load_candidate<MODEL>(a.params, on[c] ? q0 + c : q0);
+        }
+        float S[CPL], stk[CPL];
         unsigned rej_q = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, k, 0.0f, active, seg_lds[wave],
-                                                    &S, &stk, &rej_q);
+        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL>(e, t0, t0sq, k, on, 0.0f, active,
+                                                           seg_lds[wave], S, stk, &rej_q);
         if (active) {
             rejected += rej_q;
-            if (a.sembl) a.sembl[((size_t)g * ns + s) * a.P + q] = S;
-            if (a.stack) a.stack[((size_t)g * ns + s) * a.P + q] = stk;
-            if (S > best) {
-                best = S;
-                bstack = stk;
-                bidx = q;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                if (!on[c]) continue;
+                const int q = q0 + c;
+                if (a.sembl) a.sembl[((size_t)g * ns + s) * a.P + q] = S[c];
+                if (a.stack) a.stack[((size_t)g * ns + s) * a.P + q] = stk[c];
+                if (S[c] > best) {
+                    best = S[c];
+                    bstack = stk[c];
+                    bidx = q;
+                }
             }
         }
     }
@@ -143,16 +157,22 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
 
 // Host launcher for one model: picks the window/variant/storage instantiation.
 template <int MODEL>
-int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream) {
+int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+                hipStream_t stream) {
     int st = HSS_OK;
     const unsigned blocks = (unsigned)n_mid * (unsigned)a.ntiles;
     if (blocks == 0) return HSS_OK;
     int rc = dispatch_storage(storage, [&](auto S) {
         st = dispatch_variant(variant, [&](auto V) {
             dispatch_window(a.e.w, fast_index_verified(a.e.dt, a.e.tau, stream), [&](auto Wc) {
-                hipLaunchKernelGGL((grid_semblance_kernel<MODEL, decltype(Wc)::value,
-                                                          decltype(V)::value, decltype(S)::value>),
-                                   dim3(blocks), dim3(kGridThreads), 0, stream, a);
+                constexpr int WW = decltype(Wc)::value, VV = decltype(V)::value,
+                              SS = decltype(S)::value;
+                if (cpl == 2 && WW > 0)
+                    hipLaunchKernelGGL((grid_semblance_kernel<MODEL, WW, VV, SS, 2>), dim3(blocks),
+                                       dim3(kGridThreads), 0, stream, a);
+                else
+                    hipLaunchKernelGGL((grid_semblance_kernel<MODEL, WW, VV, SS, 1>), dim3(blocks),
+                                       dim3(kGridThreads), 0, stream, a);
             });
         });
     });
@@ -161,8 +181,11 @@ int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, hipStrea
     return hip_status(hipGetLastError());
 }
 
-int launch_grid_cmp(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
-int launch_grid_zocrs(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
-int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, hipStream_t stream);
+int launch_grid_cmp(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+                    hipStream_t stream);
+int launch_grid_zocrs(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+                    hipStream_t stream);
+int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+                    hipStream_t stream);
 
 }  // namespace hss

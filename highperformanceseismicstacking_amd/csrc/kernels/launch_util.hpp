@@ -2,6 +2,8 @@
 // compile-time window-size dispatch.
 #pragma once
 
+#include <functional>
+
 #include <hip/hip_runtime.h>
 
 #include "../../../include/hss.h"
@@ -39,6 +41,11 @@ inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; 
 // tau on the current device: tau >= 1 and the exhaustive check of verify.hip found no
 // mismatch (run once per device and dt, waiting on `stream` the first time).
 bool fast_index_verified(float dt, int tau, hipStream_t stream);
+
+// Grid-search parameter tile (candidates per lane) for this key, tuned on the first large
+// launch by timing launch(cpl) for each choice (tune.hip).
+int tuned_grid_cpl(int model, int w, int variant, int storage, unsigned blocks, hipStream_t stream,
+                   const std::function<int(int)> &launch, int *cpl_out);
 
 // Calls f(std::integral_constant<int, W>) for the compiled window sizes; any other w
 // (odd, <= 32), or fast = false, goes to the runtime-w instantiation W = 0.
@@ -84,6 +91,7 @@ inline int dispatch_variant(int variant, F &&f) {
     switch (variant) {
         case HSS_EXACT: f(std::integral_constant<int, HSS_EXACT>{}); return HSS_OK;
         case HSS_FAST: f(std::integral_constant<int, HSS_FAST>{}); return HSS_OK;
+        case HSS_KAHAN: f(std::integral_constant<int, HSS_KAHAN>{}); return HSS_OK;
         default: return HSS_ERR_INVALID_VARIANT;
     }
 }

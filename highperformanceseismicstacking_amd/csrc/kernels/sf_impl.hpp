@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
     const float *vp = a.vparams + (size_t)g * NP * ns;
 
-    __shared__ float seg_lds[kGridWaves][kSeg];
+    __shared__ float seg_lds[kGridWaves][kSegFloats];
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;

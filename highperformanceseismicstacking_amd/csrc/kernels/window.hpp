@@ -10,25 +10,69 @@
 //             of the reference text bit for bit.
 //   HSS_FAST  u = fma(y1 - y0, frac, y0); A_j += u; den = fma(u, u, den); the linear sum
 //             is recovered at the end as sum_j A_j (same value up to rounding).
+//   HSS_KAHAN u as HSS_FAST; every A_j and den carry a Kahan compensation term
+//             (den's also takes the rounding error of u*u, through the FMA that forms
+//             u*u - c), and the final sums over j are compensated too.
 #pragma once
 
 #include "device_common.hpp"
 
 namespace hss {
 
+// Kahan step: s + c' = s + c + x with the low part of the sum kept in c (c holds the
+// negated error, as in Kahan's original form).
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 template <int W, int VARIANT>
 struct Window {
     static constexpr int N = W > 0 ? W : kMaxWindow;
+    static constexpr bool KAHAN = VARIANT == HSS_KAHAN;
     float A[N];
-    float lin, den;
+    float Ac[KAHAN ? N : 1];
+    float lin, den, denc;
     unsigned used;
 
     __device__ __forceinline__ void reset() {
 #pragma unroll
         for (int j = 0; j < N; ++j) A[j] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < (KAHAN ? N : 1); ++j) Ac[j] = 0.0f;
         lin = 0.0f;
         den = 0.0f;
+        denc = 0.0f;
         used = 0;
+    }
+
+    // One interpolated sample u = y + frac*d at window position j, d = y_next - y (the
+    // same FP32 difference the reference forms, so HSS_EXACT keeps its bits).
+    __device__ __forceinline__ void step_d(int j, float y, float d, float f) {
+        if constexpr (VARIANT == HSS_EXACT) {
+            float u = d * f + y;
+            A[j] += u;
+            lin += u;
+            den += u * u;
+        } else if constexpr (VARIANT == HSS_FAST) {
+            float u = __builtin_fmaf(d, f, y);
+            A[j] += u;
+            den = __builtin_fmaf(u, u, den);
+        } else {
+            float u = __builtin_fmaf(d, f, y);
+            kahan_add(A[j], Ac[j], u);
+            // den + u*u with compensation: y' = u*u - c in one rounding
+            const float yy = __builtin_fmaf(u, u, -denc);
+            const float t = den + yy;
+            denc = (t - den) - yy;
+            den = t;
+        }
+    }
+
+    __device__ __forceinline__ void step(int j, float y0, float y1, float f) {
+        step_d(j, y0, y1 - y0, f);
     }
 
     // Adds the w interpolated values starting at sample `base` of the flat sample array.
@@ -43,16 +87,7 @@ struct Window {
             float y0 = y1;
             y1 = rd(base + j + 1);
             if (scaled) y1 = y1 * scale;
-            if constexpr (VARIANT == HSS_EXACT) {
-                float u = (y1 - y0) * f + y0;
-                A[j] += u;
-                lin += u;
-                den += u * u;
-            } else {
-                float u = __builtin_fmaf(y1 - y0, f, y0);
-                A[j] += u;
-                den = __builtin_fmaf(u, u, den);
-            }
+            step(j, y0, y1, f);
         }
         used++;
     }
@@ -63,18 +98,9 @@ struct Window {
 #pragma unroll
         for (int j = 0; j < N; ++j) {
             if (W == 0 && j >= w) continue;
-            float y0 = y1;
+            const float y0 = y1;
             y1 = row[j + 1];
-            if constexpr (VARIANT == HSS_EXACT) {
-                float u = (y1 - y0) * f + y0;
-                A[j] += u;
-                lin += u;
-                den += u * u;
-            } else {
-                float u = __builtin_fmaf(y1 - y0, f, y0);
-                A[j] += u;
-                den = __builtin_fmaf(u, u, den);
-            }
+            step(j, y0, y1, f);
         }
         used++;
     }
@@ -84,20 +110,34 @@ struct Window {
     __device__ __forceinline__ void reduce(int w, float *S, float *stack) const {
         float sem = 0.0f, stk = 0.0f;
         if (used > 0) {
-            float num = 0.0f;
-            float l = lin;
-            if constexpr (VARIANT == HSS_FAST) l = 0.0f;
+            float num = 0.0f, numc = 0.0f;
+            float l = lin, lc = 0.0f;
+            if constexpr (VARIANT != HSS_EXACT) l = 0.0f;
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 if (W == 0 && j >= w) continue;
                 if constexpr (VARIANT == HSS_EXACT) {
                     num += A[j] * A[j];
-                } else {
+                } else if constexpr (VARIANT == HSS_FAST) {
                     num = __builtin_fmaf(A[j], A[j], num);
                     l += A[j];
+                } else {
+                    // A_j carries its compensation: use the corrected value
+                    const float a = A[j] - Ac[j];
+                    const float y = __builtin_fmaf(a, a, -numc);
+                    const float t = num + y;
+                    numc = (t - num) - y;
+                    num = t;
+                    kahan_add(l, lc, a);
                 }
             }
-            sem = num / ((float)used * den);
+            float d = den;
+            if constexpr (KAHAN) {
+                num = num - numc;
+                l = l - lc;
+                d = den - denc;
+            }
+            sem = num / ((float)used * d);
             stk = l / (float)(used * (unsigned)w);
         }
         *S = sem;

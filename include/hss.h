@@ -45,8 +45,10 @@ enum hss_storage { HSS_F32 = 0, HSS_F16 = 1, HSS_BF16 = 2 };
  *   HSS_EXACT: the reference op order, no FMA contraction -- results are bit-identical
  *              to an IEEE FP32 evaluation of common/include/gpu/interface.h:26-55;
  *   HSS_FAST : FMA-contracted interpolation and sums, stack sum derived from the
- *              window components; within 1e-5 relative of HSS_EXACT. */
-enum hss_variant { HSS_EXACT = 0, HSS_FAST = 1 };
+ *              window components; within 1e-5 relative of HSS_EXACT;
+ *   HSS_KAHAN: as HSS_FAST with Kahan-compensated window sums (A_j, sum of squares,
+ *              and the sums over the window); no reference counterpart. */
+enum hss_variant { HSS_EXACT = 0, HSS_FAST = 1, HSS_KAHAN = 2 };
 
 /* argument errors (negative statuses) */
 enum hss_status {
@@ -134,6 +136,32 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
                        float *d_result, float *d_sembl, float *d_stack,
                        unsigned long long *d_not_used, void *stream);
 
+/* The plain per-gather grid search, kept as the baseline the batched kernel is measured
+ * against: the reference's kernel structure -- per midpoint one launch with one thread per
+ * (t0, candidate) writing d_sembl/d_stack [ns][P] (scratch, reused per midpoint), then one
+ * thread per t0 picking the first strict maximum into d_result [n][(2+np)][ns].
+ * FP32 storage, HSS_EXACT arithmetic (same bits as hss_grid_semblance with HSS_EXACT).
+ * h_begin, h_count, h_m0 [n] are HOST arrays (the midpoint loop runs on the host).
+ * Restates cmp/linear_search.cu:19-115 (zocrs/, oct/ twins) as launched per CDP by
+ * CudaLinearSearchAlgorithm::computeSemblanceAtGpuForMidpoint
+ * (cuda/src/semblance/algorithm/CudaLinearSearchAlgorithm.cu:35-163). */
+int hss_grid_semblance_plain(const hss_traces *traces, int model, const float *d_params,
+                             int32_t P, int32_t n, const int32_t *h_begin,
+                             const int32_t *h_count, const float *h_m0, float h0, float apm,
+                             float *d_sembl, float *d_stack, float *d_result, void *stream);
+
+/* The launch shape the grid-search autotuner chose on the current device for (model, w,
+ * variant, storage): *cpl = candidates per lane (1 or 2), 0 when that key has not been
+ * tuned yet.  No counterpart in the reference.  hss_grid_semblance tunes on its first
+ * launch of at least 4096 workgroups for a key (outside stream capture) by timing each
+ * shape on the caller's inputs; the environment variable HSS_AUTOTUNE=0 turns it off. */
+int hss_grid_tuned_config(int model, int w, int variant, int storage, int32_t *cpl);
+
+/* Pins the grid-search launch shape for (model, w, variant, storage) on the current
+ * device: cpl = 1 or 2 candidates per lane; 0 forgets the key (the next large launch
+ * tunes it again).  Results do not depend on the shape (bit for bit). */
+int hss_grid_set_config(int model, int w, int variant, int storage, int32_t cpl);
+
 /* Bytes of device workspace hss_de_semblance needs for n_mid midpoints of ns samples
  * with populations of NP individuals (two generations of every population). */
 int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint64_t *bytes);
@@ -179,6 +207,26 @@ int hss_stretch_free(const hss_traces *traces, const hss_midpoints *mids, int mo
                      const float *d_vparams, const float *nvals, int32_t nN, int variant,
                      float *d_result, float *d_sembl, float *d_stack,
                      unsigned long long *d_not_used, void *stream);
+
+/* Semblance gradient (no counterpart in the reference): for every (midpoint g, sample s)
+ * of a batch, S, stack and dS/d(parameters) at the point held in rows 2.. of d_points
+ * [n][(2+np)][ns] (v space: velocity, then A,B or slope -- the layout of the d_result of
+ * hss_grid_semblance / hss_de_semblance, so a search result can be passed directly).
+ * Writes d_out [n][(2+np)][ns]: S, stack, dS/dv, then dS/dA, dS/dB or dS/dslope.
+ * FMA arithmetic (as HSS_FAST).  The gradient is exact for the piecewise-smooth S away
+ * from the seams where a window slides by one sample.  Points with v <= 0 give zeros. */
+int hss_semblance_gradient(const hss_traces *traces, const hss_midpoints *mids, int model,
+                           const float *d_points, float *d_out, void *stream);
+
+/* Gauss-Newton polish (no counterpart in the reference): starting from rows 2.. of
+ * d_start [n][(2+np)][ns] (e.g. a grid or DE result), `steps` Levenberg-Marquardt-damped
+ * Gauss-Newton steps per (midpoint, sample) on the residual 1 - S in box-normalised
+ * coordinates, inside [lower, upper] (host arrays, v space).  A step is kept only when S
+ * increases, so S never decreases.  Writes d_result [n][(2+np)][ns] (S, stack,
+ * parameters), which must not alias d_start.  Each step costs one gradient evaluation. */
+int hss_polish(const hss_traces *traces, const hss_midpoints *mids, int model,
+               const float *lower, const float *upper, int32_t steps,
+               const float *d_start, float *d_result, void *stream);
 
 /* OCT trace selection: which candidates are tried at every t0. */
 enum hss_oct_mode {

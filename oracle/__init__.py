@@ -54,6 +54,7 @@ def lib():
                                  vp, vp, vp, i]
         L.orc_select_traces.argtypes = [i, vp, vp, i, fl, fl, fl, i, fl, vp, i, i, vp]
         L.orc_oct_selection_samples.argtypes = [fl, fl, fl, fl, i, vp]
+        L.orc_eval_grad_f64.argtypes = [i, vp, i, vp, i, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -208,3 +209,18 @@ def oct_time(h, h0, t0, m0, m, mh, c, slope):
     t = ctypes.c_float(0)
     err = lib().orc_oct_time(h, h0, t0, m0, m, mh, c, slope, ctypes.byref(t))
     return err, t.value
+
+
+def eval_grad_f64(model, g, s, pv, f32_index=False):
+    """FP64 (S, stack, dS/dparams) at sample s for the parameter point pv (v space)
+    (oracle_grad.c).  f32_index=False: every step in double, including t/dt and the
+    sample index; True: traveltime, index, weight and trace decisions in FP32 as the
+    FP32 evaluation takes them, derivatives and sums in double."""
+    n = NPARAMS[model]
+    pv = np.ascontiguousarray(pv, dtype=np.float64)
+    S = np.zeros(1)
+    T = np.zeros(1)
+    grad = np.zeros(n)
+    lib().orc_eval_grad_f64(model, g.ref, int(s), _p(pv), int(bool(f32_index)), _p(S), _p(T),
+                            _p(grad))
+    return float(S[0]), float(T[0]), grad

@@ -189,6 +189,19 @@ static inline int orc_traveltime(int model, const orc_gather *g, int i, float t0
     return 0;
 }
 
+/* Exported forms of the two helpers above for the gradient oracle (oracle_grad.c). */
+int orc_traveltime_f32(int model, const orc_gather *g, int i, float t0, const float *prm,
+                       float *t)
+{
+    unsigned nu = 0;
+    return orc_traveltime(model, g, i, t0, prm, t, &nu);
+}
+
+int orc_window_index_f32(float t, float dt, int ns, int tau, int *k_out, float *f_out)
+{
+    return orc_window_index(t, dt, ns, tau, k_out, f_out);
+}
+
 /* One (t0, candidate) semblance in FP32: the body of computeSemblancesFor*
  * (cmp/linear_search.cu:37-80) with COMPUTE_SEMBLANCE (interface.h:26-44) and
  * REDUCE_SEMBLANCE_STACK (interface.h:46-55).  shift_n != 0 adds the stretch-free

@@ -113,6 +113,21 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
                 int NP, int G, uint64_t seed, uint32_t stream_id, float *result,
                 float *x_out, float *fx_out, int nthreads);
 
+/* FP64 semblance, stack and dS/d(parameters) at one sample s for one parameter point
+ * pv in v space (v; v,A,B; v,slope) (oracle_grad.c).  grad[np] may be NULL.
+ *   f32_index = 0: everything in double, including the traveltime, t/dt and the sample
+ *                  index -- S is then the piecewise-smooth function the gradient is
+ *                  exact for;
+ *   f32_index = 1: the traveltime, t/dt, sample index, weight and trace decisions are
+ *                  taken in FP32 exactly as orc_eval_f32 takes them (c = 4/v^2 in
+ *                  float), the traveltime derivatives and all sums in double -- the
+ *                  counterpart of a FP32 kernel, free of index disagreements at seams. */
+void orc_eval_grad_f64(int model, const orc_gather *g, int s, const double *pv, int f32_index,
+                       double *S, double *stack, double *grad);
+int orc_traveltime_f32(int model, const orc_gather *g, int i, float t0, const float *prm,
+                       float *t);
+int orc_window_index_f32(float t, float dt, int ns, int tau, int *k_out, float *f_out);
+
 /* ---- trace selection masks over a whole line
  * (cmp/common.cu:3-15, zocrs/common.cu:3-16, oct/linear_search.cu:155-192,
  *  oct/differential_evolution.cu:130-168, oct/stretch_free.cu:99-138).

@@ -107,6 +107,30 @@ def test_batch_size_does_not_change_results(cmp_line):
         assert np.array_equal(a[n][1].view(np.uint32), b[n][1].view(np.uint32)), n
 
 
+def test_polish_steps_only_raise_semblance(cmp_line):
+    """--polish-steps: the Gauss-Newton polish of every grid winner never lowers the
+    coherence, keeps velocities inside the bounds, and moves at least some of them."""
+    tmp, path, line = cmp_line
+    lo, hi, G = 1500.0, 4500.0, 32
+    base = ["--traveltime", "cmp", "--lower-bounds", lo, "--upper-bounds", hi, "--granularity", G]
+    d0 = tmp / "nopolish"
+    d1 = tmp / "polish"
+    d0.mkdir()
+    d1.mkdir()
+    out0, _ = run("single_host_linear_search", common_args(path, str(d0)) + base, tmp)
+    out1, _ = run("single_host_linear_search", common_args(path, str(d1)) + base +
+                  ["--polish-steps", 8], tmp)
+    r0 = read_results(out0, ["coherence", "velocity"], line.ns)
+    r1 = read_results(out1, ["coherence", "velocity"], line.ns)
+    c0, c1 = r0["coherence"][1], r1["coherence"][1]
+    v0, v1 = r0["velocity"][1], r1["velocity"][1]
+    live = v0 > 0
+    assert np.all(c1[live] >= c0[live] - 1e-6 * np.abs(c0[live]))
+    assert np.all((v1[live] >= np.float32(lo)) & (v1[live] <= np.float32(hi)))
+    assert np.count_nonzero(v1[live] != v0[live]) > 0.2 * np.count_nonzero(live)
+    assert np.mean(c1[live] - c0[live]) > 0
+
+
 def test_stretch_free_from_velocity_file(cmp_line):
     tmp, path, line = cmp_line
     lo, hi, G = 1500.0, 4500.0, 64

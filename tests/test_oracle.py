@@ -8,7 +8,8 @@
     stack = its amplitude, exact scaling by 2 leaves S unchanged bit for bit;
   * textbook special cases: ZO-CRS with A = B = 0 is the CMP hyperbola; OCT with
     slope = 0 and h0 = 0 is the CMP hyperbola; the grid finds a planted velocity;
-  * a brute-force FP64 restatement of the semblance definition on tiny gathers.
+  * a brute-force FP64 restatement of the semblance definition on tiny gathers;
+  * the FP64 gradient against central finite differences of the FP64 semblance.
 """
 import os
 
@@ -299,3 +300,97 @@ def test_oct_selection_samples_follow_minstd_rand0():
     # slopes continue the same engine after the 4 velocities: the 5th output 1144108930
     u4 = (1144108930 - 1) / 2147483646.0
     assert abs(out[0, 1] - (-5e-4 + u4 * 1e-3)) <= 1e-9
+
+
+def _oracle_gather_for(name):
+    line, cfg = synth.workload(name)
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    b, c, m0s = line.cmp_ranges()
+    if name == "c1":
+        sl = slice(int(b[0]), int(b[0] + c[0]))
+        return O.Gather(line.samples[sl], line.mid[sl], line.hoff[sl], line.dt, tau,
+                        m0=float(m0s[0])), cfg
+    m0 = f32(m0s[len(m0s) // 2])
+    sel = np.flatnonzero(np.abs(m0 - line.mid) <= f32(cfg["apm"]))
+    sl = slice(int(sel[0]), int(sel[-1]) + 1)
+    return O.Gather(line.samples[sl], line.mid[sl], line.hoff[sl], line.dt, tau, m0=float(m0),
+                    h0=cfg.get("h0", 0.0), apm=cfg["apm"]), cfg
+
+
+@pytest.mark.parametrize("name,model", [("c1", O.CMP), ("c3", O.ZOCRS), ("c4", O.OCT)])
+def test_fp64_gradient_matches_central_differences(name, model):
+    """dS/dparams of the FP64 oracle against central differences of its own S.  S is
+    piecewise smooth: a window slides by one sample where some trace's t/dt crosses an
+    integer, so a difference straddling such a seam is skipped (forward and backward
+    one-sided differences disagree there).  Step 1e-8 relative: far below the seam
+    spacing, far above FP64 rounding of S."""
+    g, cfg = _oracle_gather_for(name)
+    rng = np.random.default_rng(5)
+    n = O.NPARAMS[model]
+    checked = 0
+    for _ in range(30):
+        s = int(rng.integers(50, g.ns - 50))
+        if model == O.CMP:
+            pv = np.array([rng.uniform(1600.0, 4400.0)])
+        elif model == O.ZOCRS:
+            pv = np.array([rng.uniform(1600.0, 4500.0), rng.uniform(-8e-4, 8e-4),
+                           rng.uniform(-8e-7, 8e-7)])
+        else:
+            pv = np.array([rng.uniform(1600.0, 4400.0), rng.uniform(-4e-4, 4e-4)])
+        S, _, grad = O.eval_grad_f64(model, g, s, pv)
+        for m in range(n):
+            h = abs(pv[m]) * 1e-8
+            e = np.zeros(n)
+            e[m] = h
+            Sp = O.eval_grad_f64(model, g, s, pv + e)[0]
+            Sm = O.eval_grad_f64(model, g, s, pv - e)[0]
+            fwd, bwd = (Sp - S) / h, (S - Sm) / h
+            if abs(fwd - bwd) > 1e-4 * max(abs(fwd), abs(bwd)) + 1e-12 / h:
+                continue
+            cen = (Sp - Sm) / (2 * h)
+            assert abs(cen - grad[m]) <= 1e-4 * abs(grad[m]) + 1e-9 / h, (s, pv, m, grad[m], cen)
+            checked += 1
+    assert checked >= 0.8 * 30 * n
+
+
+def test_fp64_gradient_closed_form_single_trace_shift():
+    """One trace, w = 1: u = y0 + f*(y1 - y0) on a linear ramp y = a*k + b, so the
+    window value is a*t/dt + b and, with one trace and one sample, S = 1 for every v
+    (dS/dv = 0), while stack = a*t/dt + b gives the closed form of the traveltime slope."""
+    ns, dt = 200, 0.004
+    a, b = 0.01, 0.3
+    samples = (a * np.arange(ns) + b).astype(np.float32)[None, :]
+    g = O.Gather(samples, np.array([100.0], np.float32), np.array([500.0], np.float32), dt, 0,
+                 m0=100.0)
+    for v in (1800.0, 2600.0):
+        S, T, grad = O.eval_grad_f64(O.CMP, g, 60, [v])
+        assert abs(S - 1.0) < 1e-12 and abs(grad[0]) < 1e-9
+        t = np.sqrt((60 * dt) ** 2 + 4 * 500.0 ** 2 / v ** 2)
+        assert abs(T - (a * t / dt + b)) < 1e-5
+
+
+@pytest.mark.parametrize("name,model", [("c1", O.CMP), ("c3", O.ZOCRS), ("c4", O.OCT)])
+def test_fp64_gradient_f32_index_mode_matches_fp64_twin(name, model):
+    """With the FP32 index chain the gradient oracle's S is the FP64-summing twin's S
+    (same decisions, same FP64 sums) -- and its gradient is the all-FP64 one wherever no
+    trace sits at a seam."""
+    g, cfg = _oracle_gather_for(name)
+    rng = np.random.default_rng(8)
+    agree = 0
+    for _ in range(25):
+        s = int(rng.integers(50, g.ns - 50))
+        if model == O.CMP:
+            pv = np.array([rng.uniform(1600.0, 4400.0)])
+        elif model == O.ZOCRS:
+            pv = np.array([rng.uniform(1600.0, 4500.0), rng.uniform(-8e-4, 8e-4),
+                           rng.uniform(-8e-7, 8e-7)])
+        else:
+            pv = np.array([rng.uniform(1600.0, 4400.0), rng.uniform(-4e-4, 4e-4)])
+        S1, T1, g1 = O.eval_grad_f64(model, g, s, pv, f32_index=True)
+        prm = np.array([f32(4.0) / (f32(pv[0]) * f32(pv[0]))] + [f32(x) for x in pv[1:]],
+                       dtype=np.float32)
+        S2, T2 = O.eval_f64(model, g, s, prm)[:2]
+        assert abs(S1 - S2) <= 1e-12 * max(abs(S2), 1e-30) and abs(T1 - T2) <= 1e-12 * max(abs(T2), 1e-12)
+        _, _, g0 = O.eval_grad_f64(model, g, s, pv)
+        agree += np.allclose(g1, g0, rtol=1e-3, atol=1e-3 * np.abs(g0).max() + 1e-30)
+    assert agree >= 10
