@@ -27,8 +27,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c5"])
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: CMP grid search over a 500-CMP line (the headline metric); "
+                         "c3: ZO-CRS DE + Gauss-Newton polish, BF16 (line time); c4: OCT DE + "
+                         "polish, FP16; c5: CMP grid search + stretch-free, 2000 CMPs")
+    ap.add_argument("--polish-steps", type=int, default=10,
+                    help="Gauss-Newton steps after DE (c3, c4)")
+    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
+                    help="trace storage (default: f32 for c2/c5, bf16 for c3, f16 for c4)")
     ap.add_argument("--variant", default="fast", choices=["fast", "exact", "kahan"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU work budget of the oracle baseline sample (0 disables it)")
@@ -129,6 +135,152 @@ def measure_extras(args, line, tau, model, params, G, begins, counts, m0s, mp, v
     return out
 
 
+def time_region(fn, steps, warmup, stream):
+    """Average ms of fn() over `steps` runs after `warmup`, HIP events on `stream`."""
+    import torch
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            fn()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_de(args):
+    """c3 / c4: differential evolution over the whole line in one launch, then the
+    Gauss-Newton polish of every winner; one step = DE + polish (the line time)."""
+    import numpy as np
+    import torch
+
+    from highperformanceseismicstacking_amd import (F32, F16, BF16, FAST, EXACT, KAHAN, ZOCRS, OCT,
+                                                   capi, synth)
+    storage = {"f32": F32, "f16": F16, "bf16": BF16}[args.storage]
+    variant = {"fast": FAST, "exact": EXACT, "kahan": KAHAN}[args.variant]
+    line, cfg = synth.workload(args.workload)
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    w = 2 * tau + 1
+    _, _, m0s = line.cmp_ranges()
+    f32 = np.float32
+    model = ZOCRS if args.workload == "c3" else OCT
+    NP, G = cfg["population"], cfg["generations"]
+    seed = 20240601
+    if model == ZOCRS:
+        ain = f32(f32(f32(cfg["ain"]) * f32(np.pi)) / f32(180.0))
+        ua = f32(f32(f32(2.0) * f32(np.sin(ain, dtype=np.float32))) / f32(cfg["v0"]))
+        ub = f32(f32(f32(cfg["bpctg"]) * f32(4.0)) / f32(f32(cfg["lower"][0]) * f32(cfg["lower"][0])))
+        lower = [cfg["lower"][0], float(-ua), float(-ub)]
+        upper = [cfg["upper"][0], float(ua), float(ub)]
+    else:
+        lower, upper = cfg["lower"], cfg["upper"]
+    h0 = float(cfg.get("h0", 0.0))
+    apm = float(cfg["apm"])
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    stream = torch.cuda.Stream()
+    sel_ms = 0.0
+    if model == ZOCRS:
+        # |m0 - m| <= apm: a contiguous range of the (m, h)-sorted line (zocrs/common.cu:14)
+        lo_i = np.searchsorted(line.mid, m0s - f32(apm), side="left")
+        begins, counts = [], []
+        for g, m0 in enumerate(m0s):
+            b = int(lo_i[g])
+            while b > 0 and abs(f32(m0) - line.mid[b - 1]) <= f32(apm):
+                b -= 1
+            e = b
+            while e < line.ntr and abs(f32(m0) - line.mid[e]) <= f32(apm):
+                e += 1
+            begins.append(b)
+            counts.append(e - b)
+        mp = capi.DeviceMidpoints(begins, counts, m0s, h0=h0, apm=apm)
+        counts = np.array(counts)
+    else:
+        # OCT: the traces some start-population candidate reaches (device selection)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        mask = capi.select_traces(dl, OCT, m0s, apm=apm, h0=h0, oct_mode=capi.OCT_DE_START, NP=NP,
+                                  lower=lower, upper=upper, seed=seed, stream_base=0)
+        e1.record()
+        torch.cuda.synchronize()
+        sel_ms = e0.elapsed_time(e1)
+        m = mask.cpu().numpy().astype(bool)
+        idx = [np.flatnonzero(m[g]) for g in range(len(m0s))]
+        counts = np.array([len(x) for x in idx])
+        begins = np.r_[0, np.cumsum(counts)[:-1]]
+        mp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm)
+    np_ = 3 if model == ZOCRS else 2
+    nbytes = capi.de_workspace_bytes(mp.n, line.ns, model, NP)
+    ws = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device="cuda")
+    res = torch.empty((mp.n, 2 + np_, line.ns), dtype=torch.float32, device="cuda")
+    pol = torch.empty_like(res)
+
+    def de():
+        capi.de_semblance(dl, mp, model, lower, upper, NP, G, seed, variant=variant, result=res,
+                          workspace=ws)
+
+    def pl():
+        capi.polish(dl, mp, model, lower, upper, args.polish_steps, res, result=pol)
+
+    # warm-up outside timing: the first launch runs the one-time index-sequence check
+    de()
+    pl()
+    torch.cuda.synchronize()
+    de_ms = time_region(de, args.steps, args.warmup, stream)
+    pol_ms = time_region(pl, args.steps, args.warmup, stream) if args.polish_steps > 0 else 0.0
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(args.steps):
+            de()
+            if args.polish_steps > 0:
+                pl()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    resn = res.cpu().numpy()
+    poln = pol.cpu().numpy()
+    n_evals = int(mp.n) * line.ns * NP * G
+    sample_bytes = dl.sample_bytes()
+    alg_bytes = float(np.sum(counts.astype(np.float64))) * line.ns * NP * G * (w + 1) * sample_bytes
+    achieved = alg_bytes / (de_ms * 1e-3) / 1e9
+    live = resn[:, 2, :] > 0
+    out = {
+        "metric": ("semblance evals/s (t0 x individual x generation), "
+                   f"{'ZO-CRS' if model == ZOCRS else 'OCT'} differential evolution over a line, 1 GPU"),
+        "value": n_evals / (de_ms * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {'ZO-CRS' if model == ZOCRS else 'OCT'} DE, {mp.n} "
+                        f"midpoints, traces per midpoint {int(counts.min())}-{int(counts.max())} "
+                        f"(mean {counts.mean():.0f}), ns={line.ns} dt={float(line.dt)*1e3:g} ms, "
+                        f"NP={NP}, G={G}, w={w}, then {args.polish_steps} Gauss-Newton steps",
+            "storage": args.storage,
+            "variant": args.variant,
+            "line_time_s": wall,
+            "de_launch_ms": de_ms,
+            "polish_launch_ms": pol_ms,
+            "selection_ms": sel_ms,
+            "evals_per_step": n_evals,
+            "mean_semblance_de": float(resn[:, 0, :][live].mean()),
+            "mean_semblance_polished": float(poln[:, 0, :][live].mean()),
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -143,12 +295,18 @@ def main():
 
     from highperformanceseismicstacking_amd import CMP, EXACT, FAST, KAHAN, F32, F16, BF16, capi, synth
 
+    if args.storage is None:
+        args.storage = {"c3": "bf16", "c4": "f16"}.get(args.workload, "f32")
+    if args.workload in ("c3", "c4"):
+        return bench_de(args)
     storage = {"f32": F32, "f16": F16, "bf16": BF16}[args.storage]
     variant = {"fast": FAST, "exact": EXACT, "kahan": KAHAN}[args.variant]
     line, cfg = synth.workload(args.workload)
     tau = synth.tau_index(cfg["tau"], line.dt_us)
     w = 2 * tau + 1
     lo, hi, G = cfg["lower"][0], cfg["upper"][0], cfg["granularity"][0]
+    if args.workload != "c2":
+        args.cpu_seconds = 0.0  # the CPU baseline is quoted on the headline workload
     step = float(np.float32(np.float32(hi - lo) / np.float32(G)))
     begins, counts, m0s = line.cmp_ranges()
 
@@ -202,9 +360,21 @@ def main():
     launch_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
 
     extras = {}
-    if not args.no_extras:
+    if not args.no_extras and args.workload == "c2":
         extras = measure_extras(args, line, tau, CMP, params, G, begins, counts, m0s, mp, variant,
                                 launch_ms)
+    if args.workload == "c5":
+        # stretch-free re-stack of the grid winners: shifts -10..10 around each sample
+        # (StretchFreeAlgorithm.cpp:77-82), velocity per sample from the grid result
+        vpar = result[:, 2:3, :].contiguous()
+        nvals = np.arange(-10, 11, dtype=np.float32)
+
+        def sf():
+            capi.stretch_free(dl, mp, CMP, vpar, nvals, variant=variant)
+
+        sf_ms = time_region(sf, max(1, args.steps // 2), 1, stream)
+        extras["stretch_free_launch_ms"] = sf_ms
+        extras["stretch_free_evals_per_s"] = int(mp.n) * line.ns * nvals.size / (sf_ms * 1e-3)
 
     if args.calibrate_copy_mb > 0:
         nbytes = args.calibrate_copy_mb << 20
@@ -226,7 +396,8 @@ def main():
     traffic = read_traffic(args.workload, args.storage)
 
     out = {
-        "metric": "semblance evals/s (t0 x param), CMP grid search over a line, 1 GPU",
+        "metric": ("semblance evals/s (t0 x param), CMP grid search over a line, 1 GPU"
+                   + (" (then stretch-free)" if args.workload == "c5" else "")),
         "value": n_evals / (ms_per_step * 1e-3),
         "unit": "evals/s",
         "n_gpus": 1,

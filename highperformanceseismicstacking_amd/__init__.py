@@ -10,6 +10,7 @@ import os
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libhss.so")
+HOST_LIB_PATH = os.path.join(PKG_DIR, "libhss_host.so")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
 
 CMP, ZOCRS, OCT = 0, 1, 2

@@ -88,6 +88,7 @@ def lib():
         L.hss_stretch_free.argtypes = [tr, mp, ci, vp, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_semblance_gradient.argtypes = [tr, mp, ci, vp, vp, vp]
         L.hss_grid_set_config.argtypes = [ci, ci, ci, ci, i32]
+        L.hss_semblance_vjp.argtypes = [i32, i32, i32, vp, vp, vp, vp]
         L.hss_grid_tuned_config.argtypes = [ci, ci, ci, ci, ctypes.POINTER(i32)]
         L.hss_polish.argtypes = [tr, mp, ci, vp, vp, i32, vp, vp, vp]
         L.hss_select_traces.argtypes = [tr, ci, i32, vp, f32, f32,
@@ -416,3 +417,35 @@ def grid_tuned_config(model, w, variant, storage):
     check(lib().hss_grid_tuned_config(int(model), int(w), int(variant), int(storage),
                                       ctypes.byref(c)))
     return int(c.value)
+
+
+def semblance_function():
+    """torch.autograd.Function over the adjoint kernel: S = f(points), where points is a
+    [n, np, ns] tensor of parameters (v space) for every (midpoint, sample) of a batch.
+    forward returns S [n, ns]; backward multiplies the incoming gradient by dS/dpoints
+    from hss_semblance_gradient (computed in the same launch as S).  The line and the
+    midpoint batch ride along as non-differentiable arguments."""
+    import torch
+
+    class SemblanceFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, points, line, mids, model):
+            n, np_, ns = points.shape
+            rows = torch.zeros((n, 2 + np_, ns), dtype=torch.float32, device=points.device)
+            rows[:, 2:, :] = points.detach()
+            out = semblance_gradient(line, mids, model, rows)
+            ctx.save_for_backward(out)
+            ctx.shape = (n, np_, ns)
+            return out[:, 0, :].clone()
+
+        @staticmethod
+        def backward(ctx, grad_s):
+            (rows,) = ctx.saved_tensors
+            n, np_, ns = ctx.shape
+            g = grad_s.contiguous().to(torch.float32)
+            out = torch.empty((n, np_, ns), dtype=torch.float32, device=rows.device)
+            check(lib().hss_semblance_vjp(n, np_, ns, _ptr(g), _ptr(rows), _ptr(out),
+                                          _stream_handle(None)))
+            return out, None, None, None
+
+    return SemblanceFunction

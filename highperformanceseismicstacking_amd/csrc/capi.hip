@@ -85,6 +85,20 @@ static int tiles_of(int ns) { return (ns + kWave - 1) / kWave; }
 
 using namespace hss;
 
+// Vector-Jacobian product of S with respect to the parameter points:
+// out[g][m][s] = grad_S[g][s] * rows[g][2 + m][s] (rows = hss_semblance_gradient output).
+__global__ __launch_bounds__(256) void vjp_kernel(int n, int np, int ns, const float *grad_S,
+                                                  const float *rows, float *out) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long total = (unsigned long long)n * np * ns;
+    if (i >= total) return;
+    const int s = (int)(i % (unsigned long long)ns);
+    const unsigned long long gm = i / (unsigned long long)ns;
+    const int m = (int)(gm % (unsigned long long)np);
+    const int g = (int)(gm / (unsigned long long)np);
+    out[i] = grad_S[(size_t)g * ns + s] * rows[((size_t)g * (2 + np) + 2 + m) * ns + s];
+}
+
 extern "C" {
 
 int hss_abi_version(void) { return HSS_ABI_VERSION; }
@@ -326,6 +340,20 @@ int hss_polish(const hss_traces *traces, const hss_midpoints *mids, int model, c
                void *stream) {
     if (d_start == d_result) return HSS_ERR_INVALID_SIZE;
     return adjoint_common(traces, mids, model, d_start, d_result, lower, upper, steps, true, stream);
+}
+
+
+int hss_semblance_vjp(int32_t n, int32_t np, int32_t ns, const float *d_grad_S,
+                      const float *d_grad_rows, float *d_grad_points, void *stream) {
+    if (n < 0 || ns <= 0 || np < 1 || np > 3) return HSS_ERR_INVALID_SIZE;
+    if (n == 0) return HSS_OK;
+    if (!d_grad_S || !d_grad_rows || !d_grad_points) return HSS_ERR_NULL_POINTER;
+    const unsigned long long total = (unsigned long long)n * (unsigned long long)np * (unsigned long long)ns;
+    if (total > 0xffffffffULL) return HSS_ERR_INVALID_SIZE;
+    const unsigned threads = 256, blocks = (unsigned)((total + threads - 1) / threads);
+    hipLaunchKernelGGL(vjp_kernel, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, n, np, ns,
+                       d_grad_S, d_grad_rows, d_grad_points);
+    return hip_status(hipGetLastError());
 }
 
 }  // extern "C"

@@ -1,5 +1,6 @@
 // C API of the host layer (include/hss_host.h).
 #include "hss_host.h"
+#include "semblance_op.hpp"
 
 #include <cstring>
 #include <memory>
@@ -107,6 +108,19 @@ int hss_host_parse_command_line(int program, int argc, const char **argv) {
         p->parseArguments(argc, argv);
         std::unique_ptr<Traveltime> t(p->parseTraveltime());
         Logger::getInstance()->setVerbosity(LogLevel::ERROR);
+    });
+}
+
+
+int hss_host_semblance_op(const hss_traces *traces, const hss_midpoints *mids, int model,
+                          const float *d_points, const float *d_gradS, float *d_S,
+                          float *d_gradPoints, void *stream) {
+    return guarded([&] {
+        if (!traces || !mids) throw std::invalid_argument("null traces or midpoints");
+        SemblanceOp op(*traces, *mids, model, static_cast<hipStream_t>(stream));
+        op.forward(d_points, d_S);
+        op.backward(d_gradS, d_gradPoints);
+        HIP_ASSERT(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     });
 }
 

@@ -228,6 +228,14 @@ int hss_polish(const hss_traces *traces, const hss_midpoints *mids, int model,
                const float *lower, const float *upper, int32_t steps,
                const float *d_start, float *d_result, void *stream);
 
+/* Vector-Jacobian product of the semblance (no counterpart in the reference): with
+ * d_grad_rows the d_out of hss_semblance_gradient ([n][(2+np)][ns]) and d_grad_S [n][ns]
+ * the gradient of some loss with respect to S, writes
+ * d_grad_points[g][m][s] = d_grad_S[g][s] * dS/dp_m [g][s]  ([n][np][ns]) -- the
+ * backward of a torch.autograd.Function-style op whose forward is S(points). */
+int hss_semblance_vjp(int32_t n, int32_t np, int32_t ns, const float *d_grad_S,
+                      const float *d_grad_rows, float *d_grad_points, void *stream);
+
 /* OCT trace selection: which candidates are tried at every t0. */
 enum hss_oct_mode {
     HSS_OCT_PAIRS = 0,       /* d_pairs[npairs][2] {c, slope}, the same for every t0 (grid) */

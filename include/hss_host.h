@@ -1,7 +1,7 @@
 /*
  * hss_host.h -- C API of libhss_host.so, the C++17 host layer (SU reader, traveltime
  * models, command line) of the MI355X semblance backend, for bindings and tests that
- * need the host logic without a GPU.  None of these calls touches a device.
+ * need the host logic without a GPU.  Only the calls in the last section touch a device.
  *
  * Every function returns 0 on success, -1 when the host layer raised
  * std::invalid_argument, -2 for std::logic_error, -3 for std::runtime_error and -4 for
@@ -39,6 +39,17 @@ int hss_host_traveltime_bounds(int model, const float *lower, const float *upper
  * 2 (single_host_stretch_free) and builds its traveltime, without reading data or
  * opening a device. */
 int hss_host_parse_command_line(int program, int argc, const char **argv);
+
+/* ---- device calls (need a GPU) ------------------------------------------------------
+ * SemblanceOp (host/include/semblance_op.hpp), the torch.autograd.Function-shaped C++ op
+ * over the adjoint: forward(d_points [n][np][ns]) -> d_S [n][ns], then
+ * backward(d_gradS [n][ns]) -> d_gradPoints [n][np][ns].  Synchronous.  No counterpart
+ * in the reference. */
+typedef struct hss_traces hss_traces;
+typedef struct hss_midpoints hss_midpoints;
+int hss_host_semblance_op(const hss_traces *traces, const hss_midpoints *mids, int model,
+                          const float *d_points, const float *d_gradS, float *d_S,
+                          float *d_gradPoints, void *stream);
 
 #ifdef __cplusplus
 }
