@@ -143,15 +143,15 @@ __device__ __forceinline__ bool traveltime_grad(const TraceGeom &tg, float t0, f
                                                 float h0, float apm, const Candidate &k, float *t,
                                                 float *dtd) {
     if constexpr (MODEL == HSS_CMP) {
-        *t = sqrt_of<FAST>(t0sq + k.c * tg.h2);
+        bool r;
+        traveltime<MODEL, FAST>(tg, t0, t0sq, m0, h0, apm, k, t, &r);
         dtd[0] = *t > 0.0f ? 0.5f * tg.h2 / *t : 0.0f;
         return true;
     } else if constexpr (MODEL == HSS_ZOCRS) {
         const float dm = tg.m - m0;
         const float a = t0 + k.p1 * dm;
-        const float tmp = a * a + k.p2 * dm * dm + k.c * tg.h2;
-        if (!(tmp >= 0)) return false;
-        *t = sqrt_of<FAST>(tmp);
+        bool rj;
+        if (!traveltime<MODEL, FAST>(tg, t0, t0sq, m0, h0, apm, k, t, &rj)) return false;
         const float r = *t > 0.0f ? 1.0f / *t : 0.0f;
         dtd[0] = 0.5f * tg.h2 * r;
         dtd[1] = a * dm * r;

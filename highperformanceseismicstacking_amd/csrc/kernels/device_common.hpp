@@ -43,8 +43,8 @@ enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
 // root is picked from the signs of the exact residuals x - s'*s (one FMA each), the
 // correction step of the library sequence without its denormal scaling; inputs below
 // 2^-96 take sqrtf.
-__device__ __forceinline__ float sqrt_rn(float x) {
-    if (x < 0x1.0p-96f) return sqrtf(x);
+// sqrt_rn_big: the sequence alone, for inputs known to be 0 or >= 2^-96.
+__device__ __forceinline__ float sqrt_rn_big(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float s_dn = __int_as_float(__float_as_int(s) - 1);
     const float s_up = __int_as_float(__float_as_int(s) + 1);
@@ -53,6 +53,11 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     float r = r_dn <= 0.0f ? s_dn : s;
     r = r_up > 0.0f ? s_up : r;
     return r;
+}
+
+__device__ __forceinline__ float sqrt_rn(float x) {
+    if (x < 0x1.0p-96f) return sqrtf(x);
+    return sqrt_rn_big(x);
 }
 
 // t / dt from inv = RN(1/dt): q = t*inv, then one correction with the exact residual.
@@ -135,20 +140,26 @@ struct TraceGeom {
 //   ZO-CRS zocrs/linear_search.cu:83-90
 //   OCT    oct/linear_search.cu:84-98
 // t0sq is t0*t0 rounded, hoisted out of the trace loop.  FAST: square roots by sqrt_rn.
+// FAST (used only with tau >= 1 and dt >= 2^-40, see fast_index_verified()): the CMP and
+// ZO-CRS square root is the traveltime itself, so a radicand below 2^-96 -- the only
+// inputs where sqrt_rn_big may differ from sqrtf -- gives t < 2^-48 <= dt, t/dt < 1, and
+// no window with tau >= 1 accepts it either way; the small-input branch is not needed.
 template <int MODEL, bool FAST = false>
 __device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float t0sq, float m0,
                                            float h0, float apm, const Candidate &k, float *t,
                                            bool *reject) {
     *reject = false;
     if constexpr (MODEL == HSS_CMP) {
-        *t = sqrt_of<FAST>(t0sq + k.c * tg.h2);
+        if constexpr (FAST) *t = sqrt_rn_big(t0sq + k.c * tg.h2);
+        else *t = sqrtf(t0sq + k.c * tg.h2);
         return true;
     } else if constexpr (MODEL == HSS_ZOCRS) {
         float dm = tg.m - m0;
         float tmp = t0 + k.p1 * dm;
         tmp = tmp * tmp + k.p2 * dm * dm + k.c * tg.h2;
         if (tmp >= 0) {
-            *t = sqrt_of<FAST>(tmp);
+            if constexpr (FAST) *t = sqrt_rn_big(tmp);
+            else *t = sqrtf(tmp);
             return true;
         }
         return false;
@@ -199,6 +210,23 @@ __device__ __forceinline__ bool window_index(float t, float dt, float inv, Index
     const int k = (int)ti;
     *frac = ti - (float)k;
     *first = k - tau;
+    return ti > ib.lo && ti < ib.hi;
+}
+
+// The same decision returning the sample index k itself (first = k - tau).  FAST (the
+// verified short sequences, used only with tau >= 1, so an accepted t/dt is >= 1): the
+// weight is v_fract_f32(t/dt) = t/dt - floor(t/dt), the same bits as t/dt - (float)k for
+// every t/dt >= 0 (checked exhaustively on the device with the other short sequences).
+template <bool FAST = false>
+__device__ __forceinline__ bool window_k(float t, float dt, float inv, IndexBounds ib, int *k_out,
+                                         float *frac) {
+    float ti;
+    if constexpr (FAST) ti = div_rn(t, dt, inv);
+    else ti = t / dt;
+    const int k = (int)ti;
+    if constexpr (FAST) *frac = __builtin_amdgcn_fractf(ti);
+    else *frac = ti - (float)k;
+    *k_out = k;
     return ti > ib.lo && ti < ib.hi;
 }
 

@@ -155,21 +155,22 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             tg.h2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h2), jj));
             const float sc =
                 scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
-            int first[CPL];
+            int kk[CPL];
             float f[CPL];
             bool use[CPL];
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                first[c] = 0;
+                kk[c] = 0;
                 f[c] = 0.0f;
                 float t;
                 bool r;
                 bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
                 rej += (r && on[c]) ? 1u : 0u;
                 if constexpr (SHIFT) t = t + shift;
-                ok = ok && window_index<FASTIDX>(t, e.dt, e.inv_dt, e.ib, e.tau, &first[c], &f[c]);
+                ok = ok && window_k<FASTIDX>(t, e.dt, e.inv_dt, e.ib, &kk[c], &f[c]);
                 use[c] = ok && active && on[c];
             }
+            const size_t trace0 = (size_t)ti * (size_t)e.ns;
             if constexpr (USE_SEG) {
                 // Segment passes: each pass starts at the lowest window start among the
                 // outermost pending lanes and serves every pending window that lies inside
@@ -189,14 +190,15 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                 const int avail = left * e.ns;
                 unsigned long long pending = __builtin_amdgcn_ballot_w64(any);
                 while (pending) {
-                    int fmin = 0x7fffffff;
+                    int kmin = 0x7fffffff;
 #pragma unroll
-                    for (int c = 0; c < CPL; ++c) fmin = todo[c] ? min(fmin, first[c]) : fmin;
+                    for (int c = 0; c < CPL; ++c) kmin = todo[c] ? min(kmin, kk[c]) : kmin;
                     const int lo = __builtin_ctzll(pending);
                     const int hi = 63 - __builtin_clzll(pending);
-                    const int b = min(__builtin_amdgcn_readlane(fmin, lo),
-                                      __builtin_amdgcn_readlane(fmin, hi));
-                    const size_t gb = (size_t)ti * (size_t)e.ns + (size_t)b;
+                    // first sample of the segment: the lowest window start k - tau (>= 0)
+                    const int b = min(__builtin_amdgcn_readlane(kmin, lo),
+                                      __builtin_amdgcn_readlane(kmin, hi)) - e.tau;
+                    const size_t gb = trace0 + (size_t)b;
                     const int par = STORAGE == HSS_F32 ? 0 : (int)(gb & 1);
                     // largest even (16-bit) lane offset with both samples in the array
                     const int last = STORAGE == HSS_F32 ? avail - 2 - b
@@ -206,7 +208,7 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     bool more = false;
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
-                        const int rel = first[c] - b + par;
+                        const int rel = kk[c] - e.tau - b + par;
                         const bool fits = (unsigned)rel <= (unsigned)(kSeg - 1 - W);
                         if (todo[c] && fits) acc[c].add_lds(seg + rel, f[c], e.w);
                         todo[c] = todo[c] && !fits;
@@ -214,12 +216,18 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     }
                     pending = __builtin_amdgcn_ballot_w64(more);
                 }
+            } else if constexpr (STORAGE == HSS_F32) {
+                // the trace's samples shifted by -tau: window k - tau .. k + tau + 1
+                const float *row = (const float *)e.samples + trace0 - (size_t)e.tau;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    if (use[c]) acc[c].add_row(row, (unsigned)kk[c], f[c], e.w);
+                }
             } else {
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     if (use[c])
-                        acc[c].add(rd, (size_t)ti * (size_t)e.ns + (size_t)first[c], f[c], sc, scaled,
-                                   e.w);
+                        acc[c].add(rd, trace0 + (size_t)(kk[c] - e.tau), f[c], sc, scaled, e.w);
                 }
             }
         }

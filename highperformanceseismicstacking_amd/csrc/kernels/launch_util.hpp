@@ -37,9 +37,10 @@ inline int check_midpoints(const hss_midpoints *m) {
 
 inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; }
 
-// True when sqrt_rn / div_rn give the library's sample index and weight for this dt and
-// tau on the current device: tau >= 1 and the exhaustive check of verify.hip found no
-// mismatch (run once per device and dt, waiting on `stream` the first time).
+// True when sqrt_rn / div_rn / v_fract_f32 give the library's sample index and weight for
+// this dt and tau on the current device: tau >= 1, dt >= 2^-40 and the exhaustive check
+// of verify.hip found no mismatch (run once per device and dt, waiting on `stream` the
+// first time).
 bool fast_index_verified(float dt, int tau, hipStream_t stream);
 
 // Grid-search parameter tile (candidates per lane) for this key, tuned on the first large

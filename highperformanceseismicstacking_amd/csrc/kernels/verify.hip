@@ -5,9 +5,10 @@
 // tau >= 1 could accept either value: 1 <= t/dt < 2e9 (a window needs k = (int)(t/dt) >= tau,
 // window_index()).  Below that range the two may differ -- for |t| < ~2^-103 the residual
 // of div_rn is not exact, and -0 comes out as +0 -- but both values are then rejected
-// alike, so the sample index and the weight never differ.  The launchers use the short
-// sequences only for tau >= 1.  One launch walks all 2^32 bit patterns; the result is
-// cached per (device, dt).
+// alike, so the sample index and the weight never differ.  The weight of an accepted
+// index, t/dt - (float)(int)(t/dt), is taken as v_fract_f32(t/dt); that too is compared
+// for every float in [1, 2^24).  The launchers use the short sequences only for tau >= 1.
+// One launch walks all 2^32 bit patterns; the result is cached per (device, dt).
 #include <map>
 #include <mutex>
 #include <utility>
@@ -33,6 +34,11 @@ __global__ __launch_bounds__(256) void verify_index_ops_kernel(float dt, float i
         const bool ge = e >= 1.0f && e < 2.0e9f;
         const bool gf = f >= 1.0f && f < 2.0e9f;
         if (ge != gf || (ge && __float_as_uint(e) != __float_as_uint(f))) bad++;
+        // the interpolation weight of an accepted index (t/dt in [1, 2^24)): v_fract_f32
+        if (check_sqrt && x >= 1.0f && x < 0x1.0p24f) {
+            const float w_ref = x - (float)(int)x;
+            if (__float_as_uint(__builtin_amdgcn_fractf(x)) != __float_as_uint(w_ref)) bad++;
+        }
     }
     if (bad) atomicAdd(&verify_mismatches, bad);
 }
@@ -40,7 +46,9 @@ __global__ __launch_bounds__(256) void verify_index_ops_kernel(float dt, float i
 // True when the short sequences may replace sqrtf and t/dt for this (dt, tau) on the
 // current device.  The first call per (device, dt) runs the check and waits for it.
 bool fast_index_verified(float dt, int tau, hipStream_t stream) {
-    if (tau < 1) return false;
+    // tau >= 1: no window accepts t/dt < 1; dt >= 2^-40: a traveltime below 2^-48 (a
+    // radicand below 2^-96, where sqrt_rn_big may differ from sqrtf) has t/dt < 1
+    if (tau < 1 || !(dt >= 0x1.0p-40f)) return false;
     static std::mutex mu;
     static std::map<std::pair<int, unsigned>, bool> cache;
     int dev = 0;

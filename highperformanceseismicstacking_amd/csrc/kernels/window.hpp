@@ -92,6 +92,22 @@ struct Window {
         used++;
     }
 
+    // FP32 samples of one trace: row[k .. k+w] with `row` wave-uniform (the trace's first
+    // sample minus tau) and k the lane's sample index, so the loads take a scalar base and
+    // a 32-bit lane offset.
+    __device__ __forceinline__ void add_row(const float *row, unsigned k, float f, int w) {
+        const float *p = row + k;
+        float y1 = p[0];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (W == 0 && j >= w) continue;
+            const float y0 = y1;
+            y1 = p[j + 1];
+            step(j, y0, y1, f);
+        }
+        used++;
+    }
+
     // Same as add() for a window already decoded (and scaled) into LDS: row[0..w].
     __device__ __forceinline__ void add_lds(const LdsFloat *row, float f, int w) {
         float y1 = row[0];
