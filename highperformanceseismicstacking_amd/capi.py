@@ -449,3 +449,17 @@ def semblance_function():
             return out, None, None, None
 
     return SemblanceFunction
+
+
+def xcd_balanced_order(work, bins=8):
+    """Launch order of a batch's midpoints (the C++ host layer's xcd_balanced_order): sort
+    by work, deal to `bins` bins in snake order, concatenate -- every XCD then receives
+    the same number of midpoints with about the same total work, largest first."""
+    import numpy as np
+    work = np.asarray(work)
+    idx = np.argsort(-work, kind="stable")
+    out = [[] for _ in range(bins)]
+    for k, i in enumerate(idx):
+        rnd, pos = divmod(k, bins)
+        out[bins - 1 - pos if rnd & 1 else pos].append(int(i))
+    return np.array([i for b in out for i in b], dtype=np.int64)

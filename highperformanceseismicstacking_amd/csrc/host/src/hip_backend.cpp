@@ -205,6 +205,27 @@ static std::vector<std::vector<int32_t>> masks_to_lists(const DeviceBuffer &mask
 
 // Statistics of one midpoint of a batch: its share of the batch's time, in proportion to
 // its (selected traces x evaluations) work.
+// Launch order of a batch's midpoints.  The kernels hand (midpoint, tile) workgroups to
+// the 8 XCDs in contiguous eighths of the launch (logical_block in the kernels), and the
+// folds of a line differ; sorting the midpoints by work and dealing them to 8 bins in
+// snake order (0..7, 7..0, ...) gives every XCD the same number of midpoints with about
+// the same total work, each bin largest first.  Returns the permutation of [0, n).
+static std::vector<size_t> xcd_balanced_order(const std::vector<int> &work) {
+    const size_t n = work.size(), bins = 8;
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), size_t{0});
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+    std::vector<std::vector<size_t>> bin(bins);
+    for (size_t k = 0; k < n; k++) {
+        const size_t round = k / bins, pos = k % bins;
+        bin[(round & 1) ? bins - 1 - pos : pos].push_back(idx[k]);
+    }
+    std::vector<size_t> order;
+    order.reserve(n);
+    for (auto &b : bin) order.insert(order.end(), b.begin(), b.end());
+    return order;
+}
+
 static std::array<float, kStatisticCount> batch_statistics(double work, double totalWork,
                                                            double notUsed, double kernelSec,
                                                            double selectionSec, unsigned selected,
@@ -360,7 +381,21 @@ void HipLinearSearchAlgorithm::computeSemblanceAndParametersForMidpoints(
     }
     std::vector<int> counts(m0s.size());
     for (size_t g = 0; g < m0s.size(); g++) counts[g] = batch.count(g);
-    if (launchM0.size() != m0s.size()) selectBatch(launchM0);
+    {
+        // one launch for all of them, in an order that balances the XCDs
+        std::vector<int> work(launchM0.size());
+        for (size_t j = 0; j < launchM0.size(); j++) work[j] = counts[slot[j]];
+        const std::vector<size_t> order = xcd_balanced_order(work);
+        std::vector<float> m(launchM0.size());
+        std::vector<size_t> sl(slot.size());
+        for (size_t j = 0; j < order.size(); j++) {
+            m[j] = launchM0[order[j]];
+            sl[j] = slot[order[j]];
+        }
+        launchM0.swap(m);
+        slot.swap(sl);
+    }
+    selectBatch(launchM0);
     out.assign(m0s.size(), MidpointOutput{});
     std::vector<unsigned long long> nu(launchM0.size(), 0);
     std::vector<float> res(static_cast<size_t>(launchM0.size()) * nres * ns, 0.0f);
