@@ -459,7 +459,8 @@ def xcd_balanced_order(work, bins=8):
     work = np.asarray(work)
     idx = np.argsort(-work, kind="stable")
     out = [[] for _ in range(bins)]
+    full = len(idx) // bins  # the last partial round fills bins 0.. (larger bins first)
     for k, i in enumerate(idx):
         rnd, pos = divmod(k, bins)
-        out[bins - 1 - pos if rnd & 1 else pos].append(int(i))
+        out[bins - 1 - pos if (rnd & 1) and rnd < full else pos].append(int(i))
     return np.array([i for b in out for i in b], dtype=np.int64)

@@ -216,9 +216,11 @@ static std::vector<size_t> xcd_balanced_order(const std::vector<int> &work) {
     std::iota(idx.begin(), idx.end(), size_t{0});
     std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
     std::vector<std::vector<size_t>> bin(bins);
+    const size_t full = n / bins;  // complete rounds; the last partial one fills bins 0.. so
+                                   // that the larger bins come first, as the XCD split does
     for (size_t k = 0; k < n; k++) {
         const size_t round = k / bins, pos = k % bins;
-        bin[(round & 1) ? bins - 1 - pos : pos].push_back(idx[k]);
+        bin[(round & 1) && round < full ? bins - 1 - pos : pos].push_back(idx[k]);
     }
     std::vector<size_t> order;
     order.reserve(n);

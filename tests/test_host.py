@@ -146,3 +146,27 @@ def test_command_line_errors(host):
                  "--upper-bounds", "6000", "--granularity", "10") == -2
     assert parse(host, 0, *BASE, *ok, "--granularity", "10", "--aph", "5") == -1
     assert parse(host, 0, *BASE, *ok, "--granularity", "ten") == -1
+
+
+def test_xcd_balanced_order_is_a_balanced_permutation():
+    """The launch order of a batch: a permutation whose eight contiguous chunks -- the
+    split the kernels' workgroup order gives the XCDs (sizes n//8 + 1 for the first n%8,
+    n//8 for the rest) -- carry total work within one midpoint of each other, each chunk
+    largest first."""
+    from highperformanceseismicstacking_amd import capi
+    rng = np.random.default_rng(0)
+    for n in (1, 7, 8, 9, 17, 500, 2001):
+        work = rng.integers(24, 121, size=n)
+        order = capi.xcd_balanced_order(work)
+        assert sorted(order.tolist()) == list(range(n))
+        q, r = divmod(n, 8)
+        chunks, start = [], 0
+        for x in range(8):
+            sz = q + (1 if x < r else 0)
+            chunks.append(work[order[start:start + sz]])
+            start += sz
+        sums = [int(c.sum()) for c in chunks if len(c)]
+        if n >= 16:
+            assert max(sums) - min(sums) <= int(work.max())
+        for c in chunks:
+            assert np.all(np.diff(c) <= 0)
