@@ -67,9 +67,10 @@ def lib():
     """The loaded libhss.so (raises HssError when it is not built)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise HssError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
-        L = ctypes.CDLL(LIB_PATH)
+        path = os.environ.get("HSS_LIBRARY", LIB_PATH)  # another build of the same ABI
+        if not os.path.exists(path):
+            raise HssError(f"{path} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
         vp, i32, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float
         ci, u64, u32 = ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32
         tr, mp = ctypes.POINTER(hss_traces), ctypes.POINTER(hss_midpoints)
