@@ -24,7 +24,10 @@
 
 namespace hss {
 
-constexpr int kDeWaves = 4;
+constexpr int kDeWaves = 8;
+// Every lane carries its own individual, so a trace's windows spread over far more than
+// one wave segment; DE reads 16-bit windows per lane (Window::add16) instead.
+constexpr bool kDeSegments = false;
 constexpr int kDeThreads = kDeWaves * kWave;
 constexpr float kDeF = 0.85f;   // F_FAC, common/include/gpu/interface.h:5
 constexpr float kDeCR = 0.5f;   // CR, common/include/gpu/interface.h:6
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
         for (int d = 0; d < D; ++d) x[d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
         float S, stk;
         unsigned rej = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(x), 0.0f,
-                                                    active, seg_lds[wave], &S, &stk, &rej);
+        evaluate<MODEL, W, VARIANT, STORAGE, false, kDeSegments>(
+            e, t0, t0sq, de_candidate<MODEL>(x), 0.0f, active, seg_lds[wave], &S, &stk, &rej);
         if (active) rejected += rej;
 #pragma unroll
         for (int d = 0; d < D; ++d) *rec(0, i, d) = x[d];
@@ -174,9 +177,8 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
             }
             float S, stk;
             unsigned rej = 0;
-            evaluate<MODEL, W, VARIANT, STORAGE, false>(e, t0, t0sq, de_candidate<MODEL>(trial),
-                                                        0.0f, active, seg_lds[wave], &S, &stk,
-                                                        &rej);
+            evaluate<MODEL, W, VARIANT, STORAGE, false, kDeSegments>(
+                e, t0, t0sq, de_candidate<MODEL>(trial), 0.0f, active, seg_lds[wave], &S, &stk, &rej);
             if (active) rejected += rej;
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
             const bool take = S > *rec(cur, i, D);

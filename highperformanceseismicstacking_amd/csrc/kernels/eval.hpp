@@ -162,42 +162,42 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             for (int c = 0; c < CPL; ++c) {
                 kk[c] = 0;
                 f[c] = 0.0f;
-                float t;
+                float t = 0.0f;  // defined for rejected traces too (its index is ignored)
                 bool r;
-                bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
-                rej += (r && on[c]) ? 1u : 0u;
+                const bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
+                rej += (r & on[c]) ? 1u : 0u;
                 if constexpr (SHIFT) t = t + shift;
-                ok = ok && window_k<FASTIDX>(t, e.dt, e.inv_dt, e.ib, &kk[c], &f[c]);
-                use[c] = ok && active && on[c];
+                // (non-short-circuit: the index of a rejected trace is computed and ignored,
+                // which keeps the lane predicates as masks instead of branches)
+                const bool in = window_k<FASTIDX>(t, e.dt, e.inv_dt, e.ib, &kk[c], &f[c]);
+                use[c] = ok & in & active & on[c];
             }
             const size_t trace0 = (size_t)ti * (size_t)e.ns;
             if constexpr (USE_SEG) {
-                // Segment passes: each pass starts at the lowest window start among the
+                // Segment passes: a pass starts at the lowest window start among the
                 // outermost pending lanes and serves every pending window that lies inside
-                // the segment (normally all of them, so one pass for all CPL candidates).
-                // The window that set the start always fits, so every pass makes progress.
-                bool todo[CPL];
+                // the segment.  The first pass is written out on its own (normally it
+                // serves everything); further passes loop until nothing is pending.  The
+                // window that set the start always fits, so every pass makes progress.
                 bool any = false;
 #pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    todo[c] = use[c];
-                    any = any || use[c];
-                }
+                for (int c = 0; c < CPL; ++c) any = any | use[c];
+                unsigned long long pending = __builtin_amdgcn_ballot_w64(any);
+                if (pending == 0) continue;
                 // samples from the start of trace ti to the end of the array, capped so
                 // the 32-bit products below cannot overflow (only the cap's lower bound
                 // matters: past kSeg + 2 samples the clamp is inactive)
                 const int left = min(e.ntr - ti, e.left_cap);
                 const int avail = left * e.ns;
-                unsigned long long pending = __builtin_amdgcn_ballot_w64(any);
-                while (pending) {
-                    int kmin = 0x7fffffff;
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) kmin = todo[c] ? min(kmin, kk[c]) : kmin;
-                    const int lo = __builtin_ctzll(pending);
-                    const int hi = 63 - __builtin_clzll(pending);
+                // one pass from window start k0 + tau, serving the windows in `serve`;
+                // returns which of them did not fit
+                auto pass = [&](int kmin_lane_value, unsigned long long live, const bool (&serve)[CPL],
+                                bool (&rest)[CPL]) {
+                    const int lo = __builtin_ctzll(live);
+                    const int hi = 63 - __builtin_clzll(live);
                     // first sample of the segment: the lowest window start k - tau (>= 0)
-                    const int b = min(__builtin_amdgcn_readlane(kmin, lo),
-                                      __builtin_amdgcn_readlane(kmin, hi)) - e.tau;
+                    const int b = min(__builtin_amdgcn_readlane(kmin_lane_value, lo),
+                                      __builtin_amdgcn_readlane(kmin_lane_value, hi)) - e.tau;
                     const size_t gb = trace0 + (size_t)b;
                     const int par = STORAGE == HSS_F32 ? 0 : (int)(gb & 1);
                     // largest even (16-bit) lane offset with both samples in the array
@@ -205,14 +205,34 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                                                         : avail - 2 + e.nsamp_odd - b + par;
                     load_segment<STORAGE>(e, gb - (size_t)par, min(last, 2 * kWave), sc, scaled,
                                           seg, lane);
-                    bool more = false;
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
                         const int rel = kk[c] - e.tau - b + par;
                         const bool fits = (unsigned)rel <= (unsigned)(kSeg - 1 - W);
-                        if (todo[c] && fits) acc[c].add_lds(seg + rel, f[c], e.w);
-                        todo[c] = todo[c] && !fits;
-                        more = more || todo[c];
+                        if (serve[c] & fits) acc[c].add_lds(seg + rel, f[c], e.w);
+                        rest[c] = serve[c] & !fits;
+                    }
+                };
+                int kmin = 0x7fffffff;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) kmin = use[c] ? min(kmin, kk[c]) : kmin;
+                bool todo[CPL];
+                pass(kmin, pending, use, todo);
+                bool more = false;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) more = more | todo[c];
+                pending = __builtin_amdgcn_ballot_w64(more);
+                while (pending) {
+                    int km = 0x7fffffff;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) km = todo[c] ? min(km, kk[c]) : km;
+                    bool nxt[CPL];
+                    pass(km, pending, todo, nxt);
+                    more = false;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        todo[c] = nxt[c];
+                        more = more | nxt[c];
                     }
                     pending = __builtin_amdgcn_ballot_w64(more);
                 }
@@ -222,6 +242,14 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     if (use[c]) acc[c].add_row(row, (unsigned)kk[c], f[c], e.w);
+                }
+            } else if constexpr (W > 0) {
+                // 16-bit samples, per-lane reads of whole words (add16)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    if (use[c])
+                        acc[c].template add16<STORAGE>((const uint16_t *)e.samples,
+                                                       trace0 + (size_t)(kk[c] - e.tau), f[c], sc);
                 }
             } else {
 #pragma unroll

@@ -108,6 +108,43 @@ struct Window {
         used++;
     }
 
+    // 16-bit samples read per lane without LDS (compiled window sizes only): the
+    // (W+1)/2 + 1 aligned 32-bit words around samples e0 .. e0+W, shifted by one sample
+    // when e0 is odd (v_alignbit), decoded and scaled.  Words q[0 .. NW-2] cover samples
+    // a .. a+W (a = e0 rounded down to even), all inside the window or the sample just
+    // before it; the last word is needed only for odd e0, where it holds sample e0+W (and
+    // the sample after it, inside the same aligned word); for even e0 it re-reads q[NW-2].
+    template <int STORAGE>
+    __device__ __forceinline__ void add16(const uint16_t *samples16, size_t e0, float f,
+                                          float sc) {
+        static_assert(W > 0, "add16 needs a compiled window size");
+        constexpr int NW = (W + 1) / 2 + 1;
+        const unsigned p = (unsigned)(e0 & 1);
+        const uint32_t *q = (const uint32_t *)(samples16 + (e0 - p));
+        uint32_t D[NW];
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) D[i] = q[i];
+        D[NW - 1] = q[NW - 2 + p];
+        float y[W + 1];
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) {
+            const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
+            float lo, hi;
+            if constexpr (STORAGE == HSS_F16) {
+                lo = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
+                hi = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
+            } else {
+                lo = __uint_as_float(e << 16);
+                hi = __uint_as_float(e & 0xffff0000u);
+            }
+            y[2 * i] = lo * sc;
+            y[2 * i + 1] = hi * sc;
+        }
+#pragma unroll
+        for (int j = 0; j < W; ++j) step(j, y[j], y[j + 1], f);
+        used++;
+    }
+
     // Same as add() for a window already decoded (and scaled) into LDS: row[0..w].
     __device__ __forceinline__ void add_lds(const LdsFloat *row, float f, int w) {
         float y1 = row[0];
