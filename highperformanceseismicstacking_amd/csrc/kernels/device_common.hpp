@@ -40,10 +40,12 @@ enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
 // library operations otherwise (see fast_index_verified()).
 //
 // sqrt: v_sqrt_f32 is within one ulp; of s and its two neighbours the correctly rounded
-// root is picked from the signs of the exact residuals x - s'*s (one FMA each), the
-// correction step of the library sequence without its denormal scaling; inputs below
-// 2^-96 take sqrtf.
-// sqrt_rn_big: the sequence alone, for inputs known to be 0 or >= 2^-96.
+// root is picked from the signs of the exact residuals x - s'*s (one FMA each).
+// sqrt_rn_big: that sequence alone, for inputs known to be 0 or >= 2^-96 (below, the
+// residual products can underflow).
+// sqrt_rn: inputs below 2^-96 (negative ones included) are first scaled by 2^64, exactly,
+// into that range, and the root by 2^-32 back (exact: the root of a positive float is
+// >= 2^-75, a normal number), without a branch.
 __device__ __forceinline__ float sqrt_rn_big(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float s_dn = __int_as_float(__float_as_int(s) - 1);
@@ -56,8 +58,9 @@ __device__ __forceinline__ float sqrt_rn_big(float x) {
 }
 
 __device__ __forceinline__ float sqrt_rn(float x) {
-    if (x < 0x1.0p-96f) return sqrtf(x);
-    return sqrt_rn_big(x);
+    const bool small = x < 0x1.0p-96f;
+    const float r = sqrt_rn_big(__builtin_amdgcn_ldexpf(x, small ? 64 : 0));
+    return __builtin_amdgcn_ldexpf(r, small ? -32 : 0);
 }
 
 // t / dt from inv = RN(1/dt): q = t*inv, then one correction with the exact residual.
@@ -173,6 +176,104 @@ __device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float 
         err = oct_time<FAST>(tg.h, h0, t0, m0, tg.m, mh, k.c, k.p1, t);
         return err == kNoError;
     }
+}
+
+// The OCT traveltime split by what each term depends on, for loops over (trace,
+// candidate) pairs.  OctTrace holds the terms of one trace (its half offset h and the
+// run's h0): a wave computes them once per trace instead of once per candidate and lane.
+// OctCand holds the terms of one candidate at one t0.  Every term is the expression the
+// reference spells (oct/common.cuh:7-94), with its operations in the same order as in
+// oct_displaced_midpoint() and oct_time() above, so oct_traveltime() returns the same
+// bits and the same decisions as traveltime<HSS_OCT>().
+struct OctTrace {
+    float h, m;
+    float gamma_sq;  // gamma*gamma, gamma = 2*sqrt(h*h + h0*h0)
+    float p16;       // 16*h*h*h0*h0, left to right
+    float hd;        // h*h - h0*h0
+    float e1, e2;    // (h+h0)*(h+h0), (h-h0)*(h-h0)
+    float f4;        // 4*h*h, left to right
+    int branch;      // 1: |h| > |h0|; 2: |h| < |h0|; 0: equal
+};
+
+template <bool FAST>
+__device__ __forceinline__ OctTrace oct_trace(float h, float m, float h0) {
+    OctTrace o;
+    o.h = h;
+    o.m = m;
+    const float gamma = 2.0f * sqrt_of<FAST>(h * h + h0 * h0);
+    o.gamma_sq = gamma * gamma;
+    o.p16 = 16.0f * h * h * h0 * h0;
+    o.hd = h * h - h0 * h0;
+    o.e1 = (h + h0) * (h + h0);
+    o.e2 = (h - h0) * (h - h0);
+    o.f4 = 4.0f * h * h;
+    o.branch = fabsf(h) > fabsf(h0) ? 1 : (fabsf(h) < fabsf(h0) ? 2 : 0);
+    return o;
+}
+
+struct OctCand {
+    float c, slope, t0;
+    float theta, theta_sq, two_theta;
+    float tn0_sq;    // t0*t0 - c*h0*h0
+    float tq_sq;     // tn0_quad*tn0_quad, tn0_quad = tn0_sq*tn0_sq
+    float tq_th;     // tn0_quad*theta_sq
+    float two_tq;    // 2*tn0_quad
+    float q4;        // 4*h0*h0
+};
+
+__device__ __forceinline__ OctCand oct_cand(float t0, float h0, const Candidate &k) {
+    OctCand o;
+    o.c = k.c;
+    o.slope = k.p1;
+    o.t0 = t0;
+    o.theta = t0 * k.p1;
+    o.theta_sq = o.theta * o.theta;
+    o.two_theta = 2.0f * o.theta;
+    o.tn0_sq = t0 * t0 - k.c * h0 * h0;
+    const float tn0_quad = o.tn0_sq * o.tn0_sq;
+    o.tq_sq = tn0_quad * tn0_quad;
+    o.tq_th = tn0_quad * o.theta_sq;
+    o.two_tq = 2.0f * tn0_quad;
+    o.q4 = 4.0f * h0 * h0;
+    return o;
+}
+
+// traveltime<HSS_OCT>() from the split terms.
+template <bool FAST>
+__device__ __forceinline__ bool oct_traveltime(const OctTrace &tr, const OctCand &k, float m0,
+                                               float apm, float *t, bool *reject) {
+    *reject = true;
+    // displaced midpoint (oct_displaced_midpoint)
+    float arg = k.tq_sq + k.tq_th * tr.gamma_sq + tr.p16 * k.theta_sq * k.theta_sq;
+    if (arg < 0) return false;
+    arg = k.theta_sq * tr.gamma_sq + k.two_tq + 2.0f * sqrt_of<FAST>(arg);
+    if (arg == 0) return false;
+    const float mh = m0 + k.two_theta * tr.hd / sqrt_of<FAST>(arg);
+    if (fabsf(tr.m - mh) > apm) return false;
+    *reject = false;
+    // time along the trajectory (oct_time)
+    const float dmh = (mh - m0) * (mh - m0);
+    const float w1 = tr.e1 - dmh;
+    const float w2 = tr.e2 - dmh;
+    if (w1 < 0 || w2 < 0) return false;
+    const float u = sqrt_of<FAST>(w1) + sqrt_of<FAST>(w2);
+    const float chh = k.c * tr.h * tr.h;
+    float th = k.t0;
+    if (tr.branch == 1) {
+        if (u == 0.0f) return false;
+        const float a = chh + tr.f4 / (u * u) * k.tn0_sq;
+        if (a < 0) return false;
+        th = sqrt_of<FAST>(a);
+    } else if (tr.branch == 2) {
+        const float a = chh + u * u / k.q4 * k.tn0_sq;
+        if (a < 0) return false;
+        th = sqrt_of<FAST>(a);
+    }
+    const float tn_sq = th * th - chh;
+    if (th == 0.0f || k.tn0_sq == 0.0f) return false;
+    const float ah = (k.t0 * tn_sq) / (th * k.tn0_sq) * k.slope;
+    *t = th + ah * (tr.m - mh);
+    return true;
 }
 
 // Sample index and interpolation weight (COMPUTE_SEMBLANCE, interface.h:27-31):

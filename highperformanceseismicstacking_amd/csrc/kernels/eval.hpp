@@ -49,6 +49,11 @@ __device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
     return g;
 }
 
+// Lane jj's value, in a scalar register of the wave.
+__device__ __forceinline__ float bcast(float v, int jj) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), jj));
+}
+
 // Wave segments.  For one trace, the 64 lanes of a wave (consecutive t0 samples, one
 // candidate or nearby ones) read windows that overlap almost entirely: with moveout
 // dt/dt0 = t0/t <= 1 the window starts of a wave span at most 64 samples.  Instead of
@@ -136,6 +141,11 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
     for (int c = 0; c < CPL; ++c) acc[c].reset();
     unsigned rej = 0;
     const int lane = threadIdx.x & (kWave - 1);
+    OctCand oc[CPL];
+    if constexpr (MODEL == HSS_OCT) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) oc[c] = oct_cand(t0, e.h0, k[c]);
+    }
     // Traces go by in chunks of 64: lane i loads trace j0+i's index, geometry and scale
     // with one coalesced access, and the trace loop broadcasts them with v_readlane, so
     // no memory latency sits inside the per-trace loop.
@@ -146,15 +156,29 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
         const TraceGeom g_l = have ? load_geom<MODEL>(e.mid, e.hoff, e.hoff2, ti_l)
                                    : TraceGeom{0.0f, 0.0f, 0.0f};
         const float sc_l = (scaled && have) ? e.scale[ti_l] : 1.0f;
+        // OCT: the trace's own traveltime terms, once per trace (lane i: trace j0+i)
+        OctTrace o_l{};
+        if constexpr (MODEL == HSS_OCT) o_l = oct_trace<FASTIDX>(g_l.h, g_l.m, e.h0);
         const int nchunk = min(kWave, e.cnt - j0);
         for (int jj = 0; jj < nchunk; ++jj) {
             const int ti = __builtin_amdgcn_readlane(ti_l, jj);
-            TraceGeom tg;
-            tg.m = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.m), jj));
-            tg.h = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h), jj));
-            tg.h2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g_l.h2), jj));
-            const float sc =
-                scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
+            TraceGeom tg{0.0f, 0.0f, 0.0f};
+            OctTrace ot{};
+            if constexpr (MODEL == HSS_OCT) {
+                ot.h = bcast(o_l.h, jj);
+                ot.m = bcast(o_l.m, jj);
+                ot.gamma_sq = bcast(o_l.gamma_sq, jj);
+                ot.p16 = bcast(o_l.p16, jj);
+                ot.hd = bcast(o_l.hd, jj);
+                ot.e1 = bcast(o_l.e1, jj);
+                ot.e2 = bcast(o_l.e2, jj);
+                ot.f4 = bcast(o_l.f4, jj);
+                ot.branch = __builtin_amdgcn_readlane(o_l.branch, jj);
+            } else {
+                tg.m = bcast(g_l.m, jj);
+                tg.h2 = bcast(g_l.h2, jj);
+            }
+            const float sc = scaled ? bcast(sc_l, jj) : 1.0f;
             int kk[CPL];
             float f[CPL];
             bool use[CPL];
@@ -164,7 +188,11 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                 f[c] = 0.0f;
                 float t = 0.0f;  // defined for rejected traces too (its index is ignored)
                 bool r;
-                const bool ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
+                bool ok;
+                if constexpr (MODEL == HSS_OCT)
+                    ok = oct_traveltime<FASTIDX>(ot, oc[c], e.m0, e.apm, &t, &r);
+                else
+                    ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
                 rej += (r & on[c]) ? 1u : 0u;
                 if constexpr (SHIFT) t = t + shift;
                 // (non-short-circuit: the index of a rejected trace is computed and ignored,

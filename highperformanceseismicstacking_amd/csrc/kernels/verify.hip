@@ -1,6 +1,7 @@
 // Exhaustive check of the short correctly-rounded sequences of device_common.hpp.
 //
-// sqrt_rn(x) must equal sqrtf(x) for every non-negative float (and +inf), and
+// sqrt_rn(x) must equal sqrtf(x) for every non-negative float (and +inf) and be a NaN for
+// every negative one, and
 // div_rn(t, dt, RN(1/dt)) must equal t/dt for every float t wherever a window with
 // tau >= 1 could accept either value: 1 <= t/dt < 2e9 (a window needs k = (int)(t/dt) >= tau,
 // window_index()).  Below that range the two may differ -- for |t| < ~2^-103 the residual
@@ -26,8 +27,10 @@ __global__ __launch_bounds__(256) void verify_index_ops_kernel(float dt, float i
     for (unsigned long long b = blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32); b += stride) {
         const float x = __uint_as_float((unsigned)b);
         if (x != x) continue;  // NaN inputs never reach either sequence
-        if (check_sqrt && x >= 0.0f) {
-            if (__float_as_uint(sqrt_rn(x)) != __float_as_uint(sqrtf(x))) bad++;
+        if (check_sqrt) {
+            // same bits for x >= 0 (and -0); a NaN for x < 0, as sqrtf gives
+            const float a = sqrt_rn(x);
+            if (x >= 0.0f ? __float_as_uint(a) != __float_as_uint(sqrtf(x)) : a == a) bad++;
         }
         const float e = x / dt;
         const float f = div_rn(x, dt, inv);

@@ -129,19 +129,34 @@ struct Window {
 #pragma unroll
         for (int i = 0; i < NW - 1; ++i) {
             const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
-            float lo, hi;
             if constexpr (STORAGE == HSS_F16) {
-                lo = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
-                hi = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
+                y[2 * i] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
+                y[2 * i + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
             } else {
-                lo = __uint_as_float(e << 16);
-                hi = __uint_as_float(e & 0xffff0000u);
+                y[2 * i] = __uint_as_float(e << 16);
+                y[2 * i + 1] = __uint_as_float(e & 0xffff0000u);
             }
-            y[2 * i] = lo * sc;
-            y[2 * i + 1] = hi * sc;
         }
+        if constexpr (VARIANT == HSS_FAST) {
+            // the trace's scale factored out of the window: u = sc * (y0 + frac*(y1 - y0))
+            // on the stored values, A_j += sc*u and den += sc^2 * sum_j u^2
+            float dn = 0.0f;
 #pragma unroll
-        for (int j = 0; j < W; ++j) step(j, y[j], y[j + 1], f);
+            for (int j = 0; j < W; ++j) {
+                // fma(y1, 1, -y0) = y1 - y0 exactly; written so for F16 storage, where
+                // both become one mixed-precision FMA that widens the halves itself
+                const float d = __builtin_fmaf(y[j + 1], 1.0f, -y[j]);
+                const float u = __builtin_fmaf(d, f, y[j]);
+                A[j] = __builtin_fmaf(sc, u, A[j]);
+                dn = __builtin_fmaf(u, u, dn);
+            }
+            den = __builtin_fmaf(sc * sc, dn, den);
+        } else {
+#pragma unroll
+            for (int j = 0; j <= W; ++j) y[j] = y[j] * sc;
+#pragma unroll
+            for (int j = 0; j < W; ++j) step(j, y[j], y[j + 1], f);
+        }
         used++;
     }
 
