@@ -272,12 +272,14 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     if (use[c]) acc[c].add_row(row, (unsigned)kk[c], f[c], e.w);
                 }
             } else if constexpr (W > 0) {
-                // 16-bit samples, per-lane reads of whole words (add16)
+                // 16-bit samples, per-lane reads of whole words (add16) from the trace's
+                // samples shifted by -tau, as a word row and its parity
+                const size_t row = trace0 - (size_t)e.tau;
+                const unsigned rpar = (unsigned)row & 1u;
+                const uint32_t *roww = (const uint32_t *)((const uint16_t *)e.samples + (row - rpar));
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    if (use[c])
-                        acc[c].template add16<STORAGE>((const uint16_t *)e.samples,
-                                                       trace0 + (size_t)(kk[c] - e.tau), f[c], sc);
+                    if (use[c]) acc[c].template add16<STORAGE>(roww, rpar, (unsigned)kk[c], f[c], sc);
                 }
             } else {
 #pragma unroll

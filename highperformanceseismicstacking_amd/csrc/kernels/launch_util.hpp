@@ -24,6 +24,8 @@ inline int check_traces(const hss_traces *t) {
     if (t->ntr < 0 || t->ns <= 0 || t->ns > (1 << 24)) return HSS_ERR_INVALID_SIZE;
     if (t->tau < 0 || t->w != 2 * t->tau + 1 || t->w > 32) return HSS_ERR_INVALID_WINDOW;
     if (t->storage < HSS_F32 || t->storage > HSS_BF16) return HSS_ERR_INVALID_STORAGE;
+    // the kernels read samples in aligned 4-byte words (two 16-bit samples per word)
+    if (((uintptr_t)t->samples & 3u) != 0) return HSS_ERR_INVALID_STORAGE;
     if (!(t->dt > 0.0f)) return HSS_ERR_INVALID_SIZE;
     return HSS_OK;
 }

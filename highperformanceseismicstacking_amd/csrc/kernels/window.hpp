@@ -114,13 +114,18 @@ struct Window {
     // a .. a+W (a = e0 rounded down to even), all inside the window or the sample just
     // before it; the last word is needed only for odd e0, where it holds sample e0+W (and
     // the sample after it, inside the same aligned word); for even e0 it re-reads q[NW-2].
+    // The window's first sample e0 is sample rpar + k of the wave-uniform word row `roww`
+    // (the row's first sample rounded down to an even sample, i.e. a 4-byte word of the
+    // aligned sample array; rpar = 1 when it was odd), so the loads take a scalar base and
+    // a 32-bit lane offset.
     template <int STORAGE>
-    __device__ __forceinline__ void add16(const uint16_t *samples16, size_t e0, float f,
-                                          float sc) {
+    __device__ __forceinline__ void add16(const uint32_t *roww, unsigned rpar, unsigned k,
+                                          float f, float sc) {
         static_assert(W > 0, "add16 needs a compiled window size");
         constexpr int NW = (W + 1) / 2 + 1;
-        const unsigned p = (unsigned)(e0 & 1);
-        const uint32_t *q = (const uint32_t *)(samples16 + (e0 - p));
+        const unsigned e0 = rpar + k;
+        const unsigned p = e0 & 1u;
+        const uint32_t *q = roww + (e0 >> 1);
         uint32_t D[NW];
 #pragma unroll
         for (int i = 0; i < NW - 1; ++i) D[i] = q[i];

@@ -57,7 +57,7 @@ enum hss_status {
     HSS_ERR_INVALID_WINDOW = -2,   /* w != 2*tau+1, or w > 32 (MAX_WINDOW_SIZE) */
     HSS_ERR_INVALID_SIZE = -3,     /* a count is negative, zero where it must not be, or too large */
     HSS_ERR_NULL_POINTER = -4,
-    HSS_ERR_INVALID_STORAGE = -5,
+    HSS_ERR_INVALID_STORAGE = -5,  /* unknown storage, or samples not 4-byte aligned */
     HSS_ERR_INVALID_VARIANT = -6,
     HSS_ERR_NO_DEVICE = -7
 };
@@ -67,7 +67,7 @@ enum hss_status {
  * The float arrays are the GatherData MDPNT / HLFOFFST / HLFOFFST_SQ columns
  * (ComputeAlgorithm.hpp:18-27, ComputeAlgorithm.cpp:31-61). */
 typedef struct hss_traces {
-    const void  *samples;   /* [ntr][ns] in `storage`                              */
+    const void  *samples;   /* [ntr][ns] in `storage`, 4-byte aligned              */
     const float *scale;     /* [ntr] per-trace scale; NULL => 1 (required for F16) */
     const float *mid;       /* [ntr] midpoint m                                    */
     const float *hoff;      /* [ntr] signed half offset h                          */

@@ -293,3 +293,17 @@ def test_empty_midpoint_gives_zero_results():
                               params, tau, EXACT)
     assert np.all(S == 0) and np.all(T == 0)
     assert np.all(res[0, 0] == 0) and np.all(res[0, 1] == 0)
+
+
+def test_misaligned_16bit_samples_are_rejected():
+    """16-bit samples are read as aligned 4-byte words: a sample array that does not start
+    on a 4-byte boundary is refused with HSS_ERR_INVALID_STORAGE before any launch."""
+    import torch
+    line, cfg, tau, params = c1_problem()
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=F16)
+    b, c, m0 = line.cmp_ranges()
+    mp = capi.DeviceMidpoints(b, c, m0)
+    prm = torch.from_numpy(np.ascontiguousarray(params)).cuda()
+    dl.struct.samples = dl.samples.data_ptr() + 2
+    with pytest.raises(capi.HssError, match="-5"):
+        capi.grid_semblance(dl, mp, CMP, prm, params.size)
