@@ -50,10 +50,29 @@ def parse():
     return ap.parse_args()
 
 
-def read_traffic(workload, storage):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    copy-kernel calibrated), when a summary for this workload is committed."""
-    path = os.path.join(ROOT, "profiles", f"traffic_grid_{workload}_{storage}.txt")
+def calibration_copy(mb):
+    """One plain device-to-device copy of `mb` MiB, the known-size reference that scales
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters (nothing when mb is 0)."""
+    if mb <= 0:
+        return
+    import torch
+    nbytes = mb << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    # write 1 GiB elsewhere first, so no part of src is still held in a cache
+    flush = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    del flush
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def read_traffic(kernel, workload, storage):
+    """HBM bytes per launch of `kernel` ("grid", "de") measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when a summary for this workload is
+    committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}_{storage}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -242,7 +261,17 @@ def bench_de(args):
     poln = pol.cpu().numpy()
     n_evals = int(mp.n) * line.ns * NP * G
     sample_bytes = dl.sample_bytes()
-    alg_bytes = float(np.sum(counts.astype(np.float64))) * line.ns * NP * G * (w + 1) * sample_bytes
+    # gathered samples: every (t0, individual, generation) reads w+1 samples of each
+    # selected trace it keeps; OCT drops the traces outside the displaced-midpoint
+    # aperture (notUsed), counted by one more, untimed, launch
+    kept = float(np.sum(counts.astype(np.float64))) * line.ns * NP * G
+    if model == OCT:
+        nu = torch.zeros(mp.n, dtype=torch.int64, device="cuda")
+        capi.de_semblance(dl, mp, model, lower, upper, NP, G, seed, variant=variant,
+                          workspace=ws, not_used=nu)
+        torch.cuda.synchronize()
+        kept -= float(nu.sum().item())
+    alg_bytes = kept * (w + 1) * sample_bytes
     achieved = alg_bytes / (de_ms * 1e-3) / 1e9
     live = resn[:, 2, :] > 0
     out = {
@@ -275,9 +304,11 @@ def bench_de(args):
             "mean_semblance_polished": float(poln[:, 0, :][live].mean()),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic("de", args.workload, args.storage)},
         "cpu_baseline": None,
     }
+    calibration_copy(args.calibrate_copy_mb)
     print(json.dumps(out), flush=True)
 
 
@@ -381,24 +412,14 @@ def main():
         extras["stretch_free_launch_ms"] = sf_ms
         extras["stretch_free_evals_per_s"] = int(mp.n) * line.ns * nvals.size / (sf_ms * 1e-3)
 
-    if args.calibrate_copy_mb > 0:
-        nbytes = args.calibrate_copy_mb << 20
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        # write 1 GiB elsewhere first, so no part of src is still held in a cache
-        flush = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
-        torch.cuda.synchronize()
-        del flush
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    calibration_copy(args.calibrate_copy_mb)
 
     n_evals = int(mp.n) * line.ns * G
     sample_bytes = dl.sample_bytes()
     alg_bytes = float(np.sum(counts.astype(np.float64))) * line.ns * G * (w + 1) * sample_bytes
     ms_per_step = wall * 1000.0 / args.steps
     achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.workload, args.storage)
+    traffic = read_traffic("grid", args.workload, args.storage)
 
     out = {
         "metric": ("semblance evals/s (t0 x param), CMP grid search over a line, 1 GPU"

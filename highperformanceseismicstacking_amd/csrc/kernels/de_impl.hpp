@@ -12,7 +12,7 @@
 // layout the CPU oracle uses, so runs are reproducible and checkable.
 //
 // MI355X mapping: one workgroup = (midpoint, 64 consecutive t0); lane l of every wave
-// owns population s = tile*64 + l; wave v handles individuals v, v+4, ...  The
+// owns population s = tile*64 + l; wave v of V handles individuals v, v+V, ...  The
 // populations live in a caller-provided workspace, double buffered (trials of generation
 // g read generation g's individuals only), lane-fastest so every access is a 256-byte
 // coalesced row; one workgroup barrier separates generations.
@@ -24,11 +24,33 @@
 
 namespace hss {
 
-constexpr int kDeWaves = 8;
+// Launch shape, chosen by timing a ZO-CRS line (200 midpoints, ~930 traces each, NP 32,
+// G 50, BF16) and an OCT line (300 midpoints, ~2150 traces each, NP 48, G 40, FP16):
+//  * de_cpl: individuals per lane in one trace loop.  Two interleave two independent
+//    traveltime/window chains, which hides the latency of the long dependent OCT and
+//    ZO-CRS sequences better than more resident waves do (OCT -10%, ZO-CRS -6%; three
+//    were slower for ZO-CRS and no better for OCT).
+//  * de_min_occupancy: the register budget, as waves per SIMD the compiler must fit
+//    (OCT 5: 96 VGPRs; ZO-CRS 6: 80; CMP 8: 64), no spills inside the trace loop.
+//  * de_waves: waves per workgroup, so that whole workgroups fill a CU's wave slots at
+//    that occupancy (OCT 4 x 5 workgroups = 20 per CU; ZO-CRS 8 x 3 = 24; CMP 8 x 4).
+// Kahan windows (twice the accumulators), windows above 11 samples and the runtime-w
+// kernel keep one individual per lane and the compiler's own register budget.
+template <int MODEL>
+constexpr int de_waves() { return MODEL == HSS_OCT ? 4 : 8; }
+template <int MODEL, int W, int VARIANT>
+constexpr bool de_tuned() { return VARIANT != HSS_KAHAN && W != 0 && W <= 11; }
+template <int MODEL, int W, int VARIANT>
+constexpr int de_cpl() { return de_tuned<MODEL, W, VARIANT>() && MODEL != HSS_CMP ? 2 : 1; }
+template <int MODEL, int W, int VARIANT>
+constexpr int de_min_occupancy() {
+    if (!de_tuned<MODEL, W, VARIANT>()) return 1;
+    return MODEL == HSS_OCT ? 5 : (MODEL == HSS_ZOCRS ? 6 : 8);
+}
+constexpr int kDeWavesMax = 8;
 // Every lane carries its own individual, so a trace's windows spread over far more than
 // one wave segment; DE reads 16-bit windows per lane (Window::add16) instead.
 constexpr bool kDeSegments = false;
-constexpr int kDeThreads = kDeWaves * kWave;
 constexpr float kDeF = 0.85f;   // F_FAC, common/include/gpu/interface.h:5
 constexpr float kDeCR = 0.5f;   // CR, common/include/gpu/interface.h:6
 
@@ -105,7 +127,9 @@ __device__ __forceinline__ Candidate de_candidate(const float *x) {
 }
 
 template <int MODEL, int W, int VARIANT, int STORAGE>
-__global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
+__global__ __launch_bounds__(kDeWavesMax * kWave)
+__attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) void de_semblance_kernel(DeArgs a) {
+    constexpr int kDeWaves = de_waves<MODEL>();
     constexpr int D = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     constexpr int REC = D + 2;  // D parameters, S, stack
     const int lane = threadIdx.x & (kWave - 1);
@@ -131,61 +155,92 @@ __global__ __launch_bounds__(kDeThreads) void de_semblance_kernel(DeArgs a) {
     // Generation g's population lives in buffer g & 1 of the workspace, so a run split
     // over several launches ([gen_begin, gen_end) each) gives the same result as one.
     // generation 0: the start population (startPopulations) and its fitness
-    for (int i = wave; a.gen_begin == 0 && i < NP; i += kDeWaves) {
-        float u[4];
-        de_uniforms<D>(a, (uint32_t)s, stream, i, 0, u);
-        float x[D];
+    // A wave evaluates CPL individuals i0, i0+V, ... at a time (one trace loop for all);
+    // a slot past NP is evaluated on individual i0's data and discarded.
+    constexpr int CPL = de_cpl<MODEL, W, VARIANT>();
+    for (int i0 = wave; a.gen_begin == 0 && i0 < NP; i0 += kDeWaves * CPL) {
+        float x[CPL][D];
+        Candidate kc[CPL];
+        bool on[CPL];
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
-        float S, stk;
+        for (int c = 0; c < CPL; ++c) {
+            const int i = i0 + c * kDeWaves;
+            on[c] = i < NP;
+            float u[4];
+            de_uniforms<D>(a, (uint32_t)s, stream, on[c] ? i : i0, 0, u);
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[c][d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
+            kc[c] = de_candidate<MODEL>(x[c]);
+        }
+        float S[CPL], stk[CPL];
         unsigned rej = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, false, kDeSegments>(
-            e, t0, t0sq, de_candidate<MODEL>(x), 0.0f, active, seg_lds[wave], &S, &stk, &rej);
+        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments>(
+            e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
         if (active) rejected += rej;
 #pragma unroll
-        for (int d = 0; d < D; ++d) *rec(0, i, d) = x[d];
-        *rec(0, i, D) = S;
-        *rec(0, i, D + 1) = stk;
+        for (int c = 0; c < CPL; ++c) {
+            if (!on[c]) continue;
+            const int i = i0 + c * kDeWaves;
+#pragma unroll
+            for (int d = 0; d < D; ++d) *rec(0, i, d) = x[c][d];
+            *rec(0, i, D) = S[c];
+            *rec(0, i, D + 1) = stk[c];
+        }
     }
     __syncthreads();
 
     for (int gen = a.gen_begin > 1 ? a.gen_begin : 1; gen < a.gen_end; ++gen) {
         const int cur = (gen - 1) & 1;
-        for (int i = wave; i < NP; i += kDeWaves) {
-            constexpr int NQ = 4 * D + 1;
-            float u[NQ];
-            de_uniforms<NQ>(a, (uint32_t)s, stream, i, gen, u);
-            float trial[D];
-            // mutatePopulations (common/differential_evolution.cu:43-89)
-            float mutant[D];
+        for (int i0 = wave; i0 < NP; i0 += kDeWaves * CPL) {
+            float trial[CPL][D];
+            Candidate kc[CPL];
+            bool on[CPL];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const unsigned r1 = (unsigned)(u[3 * d] * (float)(NP - 1));
-                const unsigned r2 = (unsigned)(u[3 * d + 1] * (float)(NP - 1));
-                const unsigned r3 = (unsigned)(u[3 * d + 2] * (float)(NP - 1));
-                float nv = *rec(cur, r1, d) + kDeF * (*rec(cur, r2, d) - *rec(cur, r3, d));
-                nv = fminf(nv, a.hi[d]);
-                nv = fmaxf(nv, a.lo[d]);
-                mutant[d] = nv;
-            }
-            // crossoverPopulations (common/differential_evolution.cu:91-122)
-            const unsigned l = (unsigned)(u[3 * D] * (float)(D - 1));
+            for (int c = 0; c < CPL; ++c) {
+                const int i1 = i0 + c * kDeWaves;
+                on[c] = i1 < NP;
+                const int i = on[c] ? i1 : i0;
+                constexpr int NQ = 4 * D + 1;
+                float u[NQ];
+                de_uniforms<NQ>(a, (uint32_t)s, stream, i, gen, u);
+                // mutatePopulations (common/differential_evolution.cu:43-89)
+                float mutant[D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const float r = u[3 * D + 1 + d];
-                trial[d] = (r > kDeCR && l != (unsigned)d) ? *rec(cur, i, d) : mutant[d];
+                for (int d = 0; d < D; ++d) {
+                    const unsigned r1 = (unsigned)(u[3 * d] * (float)(NP - 1));
+                    const unsigned r2 = (unsigned)(u[3 * d + 1] * (float)(NP - 1));
+                    const unsigned r3 = (unsigned)(u[3 * d + 2] * (float)(NP - 1));
+                    float nv = *rec(cur, r1, d) + kDeF * (*rec(cur, r2, d) - *rec(cur, r3, d));
+                    nv = fminf(nv, a.hi[d]);
+                    nv = fmaxf(nv, a.lo[d]);
+                    mutant[d] = nv;
+                }
+                // crossoverPopulations (common/differential_evolution.cu:91-122)
+                const unsigned l = (unsigned)(u[3 * D] * (float)(D - 1));
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const float r = u[3 * D + 1 + d];
+                    trial[c][d] = (r > kDeCR && l != (unsigned)d) ? *rec(cur, i, d) : mutant[d];
+                }
+                kc[c] = de_candidate<MODEL>(trial[c]);
             }
-            float S, stk;
+            float S[CPL], stk[CPL];
             unsigned rej = 0;
-            evaluate<MODEL, W, VARIANT, STORAGE, false, kDeSegments>(
-                e, t0, t0sq, de_candidate<MODEL>(trial), 0.0f, active, seg_lds[wave], &S, &stk, &rej);
+            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments>(
+                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
             if (active) rejected += rej;
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
-            const bool take = S > *rec(cur, i, D);
 #pragma unroll
-            for (int d = 0; d < D; ++d) *rec(cur ^ 1, i, d) = take ? trial[d] : *rec(cur, i, d);
-            *rec(cur ^ 1, i, D) = take ? S : *rec(cur, i, D);
-            *rec(cur ^ 1, i, D + 1) = take ? stk : *rec(cur, i, D + 1);
+            for (int c = 0; c < CPL; ++c) {
+                if (!on[c]) continue;
+                const int i = i0 + c * kDeWaves;
+                const bool take = S[c] > *rec(cur, i, D);
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                    *rec(cur ^ 1, i, d) = take ? trial[c][d] : *rec(cur, i, d);
+                *rec(cur ^ 1, i, D) = take ? S[c] : *rec(cur, i, D);
+                *rec(cur ^ 1, i, D + 1) = take ? stk[c] : *rec(cur, i, D + 1);
+            }
         }
         __syncthreads();
     }
@@ -236,7 +291,7 @@ int launch_de(const DeArgs &a, int n_mid, int variant, int storage, hipStream_t 
             dispatch_window(a.e.w, fast_index_verified(a.e.dt, a.e.tau, stream), [&](auto Wc) {
                 hipLaunchKernelGGL((de_semblance_kernel<MODEL, decltype(Wc)::value,
                                                         decltype(V)::value, decltype(S)::value>),
-                                   dim3(blocks), dim3(kDeThreads), 0, stream, a);
+                                   dim3(blocks), dim3(de_waves<MODEL>() * kWave), 0, stream, a);
             });
         });
     });
