@@ -68,7 +68,7 @@ enum hss_status {
  * (ComputeAlgorithm.hpp:18-27, ComputeAlgorithm.cpp:31-61). */
 typedef struct hss_traces {
     const void  *samples;   /* [ntr][ns] in `storage`, 4-byte aligned              */
-    const float *scale;     /* [ntr] per-trace scale; NULL => 1 (required for F16) */
+    const float *scale;     /* [ntr] per-trace scale; NULL => 1                     */
     const float *mid;       /* [ntr] midpoint m                                    */
     const float *hoff;      /* [ntr] signed half offset h                          */
     const float *hoff2;     /* [ntr] h*h                                           */
