@@ -165,7 +165,7 @@ def test_autograd_function_drives_a_torch_optimizer():
         S = Fn.apply(p, dl, mp, CMP)
         loss = -S.mean()
         if first is None:
-            first = float(S.mean())
+            first = float(S.detach().mean())
         loss.backward()
         opt.step()
         with torch.no_grad():

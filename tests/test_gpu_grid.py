@@ -80,11 +80,12 @@ def test_short_index_sequences_verified(dt):
     assert capi.verify_index_ops(dt) == 0
 
 
-@pytest.mark.parametrize("tau", [0, 1, 2, 7])
+@pytest.mark.parametrize("tau", [0, 1, 2, 7, 15])
 def test_cmp_grid_exact_bitwise_window_edges(tau):
-    """Small and odd windows (w = 1 and 3 take the runtime-w kernel, 5 and 15 compiled
-    ones) on a short trace, so many windows cross both ends of the trace: EXACT equals the
-    oracle bit for bit, including which traces each window keeps."""
+    """Small, odd and maximal windows (w = 1, 3 and 31 take the runtime-w kernel, 5 and 15
+    compiled ones; 31 is the largest w under the reference's MAX_WINDOW_SIZE of 32) on a
+    short trace, so many windows cross both ends of the trace: EXACT equals the oracle bit
+    for bit, including which traces each window keeps."""
     line = synth.make_line(1, 24, 1000.0, 61, 4000, seed=7, offsets=np.arange(0, 2400, 100.0),
                            m_first=1000.0)
     step = grid_steps([1500.0], [4500.0], [64])
@@ -307,3 +308,37 @@ def test_misaligned_16bit_samples_are_rejected():
     dl.struct.samples = dl.samples.data_ptr() + 2
     with pytest.raises(capi.HssError, match="-5"):
         capi.grid_semblance(dl, mp, CMP, prm, params.size)
+
+
+def _nan_aware_equal(a, b):
+    """Same NaN positions, and the other values bit for bit (a NaN's payload is not
+    compared: 0/0 gives different NaN bits on the CPU and the GPU)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(a[~na].view(np.uint32), b[~nb].view(np.uint32))
+
+
+@pytest.mark.parametrize("variant", [EXACT, FAST])
+def test_muted_zone_gives_nan_that_never_wins(variant):
+    """A muted zone (all-zero samples) gives windows with used > 0 and sum(u^2) = 0, whose
+    semblance is 0/0 = NaN (interface.h:46-55); a NaN never wins the strict-'>' scan, and
+    a t0 whose every candidate is NaN reports no best candidate.  EXACT matches the
+    oracle's values and NaN positions; FAST has the same NaN positions."""
+    line = synth.make_line(2, 24, 2000.0, 301, 4000, seed=5, offset_min=50.0)
+    line.samples[:, :120] = 0.0  # muted first 0.48 s
+    step = grid_steps([1500.0], [4500.0], [48])
+    params = O.build_parameter_array(O.CMP, [1500.0], step, [48])
+    b, c, m0 = line.cmp_ranges()
+    tau = synth.tau_index(0.02, line.dt_us)
+    res, S, T, _, _ = run_gpu(line, b, c, m0, CMP, params, tau, variant)
+    for g in range(len(b)):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]))
+        rres, rS, rT, rbest, _ = O.grid_f32(O.CMP, og, params)
+        assert np.isnan(rS).any() and not np.isnan(rS).all()
+        assert np.array_equal(np.isnan(S[g]), np.isnan(rS))
+        if variant == EXACT:
+            assert _nan_aware_equal(S[g], rS)
+            assert _nan_aware_equal(res[g], rres)
+        gbest = first_max_index(S[g])
+        assert np.array_equal(gbest, first_max_index(rS))
+        assert np.array_equal(gbest, rbest)
+
