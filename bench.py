@@ -68,6 +68,21 @@ def calibration_copy(mb):
     del src, dst
 
 
+def roofline(achieved, traffic, launch_ms):
+    """The roofline object: `achieved` is the gathered-sample stream (algorithmic bytes
+    per launch / launch time, GB/s) against the HBM peak; `traffic` the HBM bytes per
+    launch the counters measured (None when no summary is committed), and
+    `traffic_gbs` that traffic over the launch time.  The gathered stream is served
+    mostly from registers, L1, L2 and LDS, so `frac` can exceed 1; the kernels are bound
+    by the vector ALU, not by HBM."""
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+    if traffic is not None:
+        out["traffic_gbs"] = traffic / (launch_ms * 1e-3) / 1e9
+    out["binding"] = "vector ALU"
+    return out
+
+
 def read_traffic(kernel, workload, storage):
     """HBM bytes per launch of `kernel` ("grid", "de") measured by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), when a summary for this workload is
@@ -303,9 +318,8 @@ def bench_de(args):
             "mean_semblance_de": float(resn[:, 0, :][live].mean()),
             "mean_semblance_polished": float(poln[:, 0, :][live].mean()),
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic("de", args.workload, args.storage)},
+        "roofline": roofline(achieved, read_traffic("de", args.workload, args.storage),
+                             de_ms),
         "cpu_baseline": None,
     }
     calibration_copy(args.calibrate_copy_mb)
@@ -448,14 +462,7 @@ def main():
             "parameter_tile": capi.grid_tuned_config(CMP, w, variant, storage),
             **extras,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
+        "roofline": roofline(achieved, traffic, launch_ms),
         "cpu_baseline": None,
     }
     if args.cpu_seconds > 0:

@@ -45,7 +45,8 @@ enum ErrorCode {  // gpu_error_code, common/include/gpu/interface.h:12-18
 // residual products can underflow).
 // sqrt_rn: inputs below 2^-96 (negative ones included) are first scaled by 2^64, exactly,
 // into that range, and the root by 2^-32 back (exact: the root of a positive float is
-// >= 2^-75, a normal number), without a branch.
+// >= 2^-75, a normal number).  The scaling runs only when some active lane of the wave
+// has such an input (a wave-uniform branch); it is the identity for the other lanes.
 __device__ __forceinline__ float sqrt_rn_big(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float s_dn = __int_as_float(__float_as_int(s) - 1);
@@ -59,6 +60,7 @@ __device__ __forceinline__ float sqrt_rn_big(float x) {
 
 __device__ __forceinline__ float sqrt_rn(float x) {
     const bool small = x < 0x1.0p-96f;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(small) == 0, 1)) return sqrt_rn_big(x);
     const float r = sqrt_rn_big(__builtin_amdgcn_ldexpf(x, small ? 64 : 0));
     return __builtin_amdgcn_ldexpf(r, small ? -32 : 0);
 }
