@@ -5,6 +5,9 @@ One step = one pass of the hot path over the line: every (midpoint, t0, velocity
 semblance of the 500-midpoint line (fold 48-96, 1501 samples at 2 ms, 1024 velocities
 1400-5000 m/s, 11-sample window) with the best-velocity pick fused in, in one launch.
 Inputs are resident in HBM before the timed region starts.  Prints ONE JSON line.
+Unless --no-extras, the line also carries the plain per-gather FP32 baseline, the FP16
+storage timing and the second part of the metric, the ZO-CRS differential-evolution line
+time (--workload c3 on its own gives that workload's full line).
 
 Roofline: the algorithmic bytes of a launch are the gathered-sample stream,
 sum over (midpoint, t0, candidate) of fold * (w+1) * sample_bytes, divided by the
@@ -186,8 +189,13 @@ def time_region(fn, steps, warmup, stream):
 
 
 def bench_de(args):
+    print(json.dumps(run_de(args)), flush=True)
+
+
+def run_de(args):
     """c3 / c4: differential evolution over the whole line in one launch, then the
-    Gauss-Newton polish of every winner; one step = DE + polish (the line time)."""
+    Gauss-Newton polish of every winner; one step = DE + polish (the line time).  Returns
+    the JSON object."""
     import numpy as np
     import torch
 
@@ -323,7 +331,7 @@ def bench_de(args):
         "cpu_baseline": None,
     }
     calibration_copy(args.calibrate_copy_mb)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -413,6 +421,16 @@ def main():
     if not args.no_extras and args.workload == "c2":
         extras = measure_extras(args, line, tau, CMP, params, G, begins, counts, m0s, mp, variant,
                                 launch_ms)
+        # the metric's second part: the ZO-CRS DE line time (c3: 200 midpoints, NP 32,
+        # G 50, BF16, then 10 Gauss-Newton steps), measured as `--workload c3` does
+        de_args = argparse.Namespace(**vars(args))
+        de_args.workload, de_args.storage, de_args.variant = "c3", "bf16", "fast"
+        de_args.steps, de_args.warmup, de_args.calibrate_copy_mb = 3, 1, 0
+        de = run_de(de_args)
+        extras["zocrs_de_line_time_s"] = de["config"]["line_time_s"]
+        extras["zocrs_de_launch_ms"] = de["config"]["de_launch_ms"]
+        extras["zocrs_de_evals_per_s"] = de["value"]
+        extras["zocrs_de_workload"] = de["config"]["workload"]
     if args.workload == "c5":
         # stretch-free re-stack of the grid winners: shifts -10..10 around each sample
         # (StretchFreeAlgorithm.cpp:77-82), velocity per sample from the grid result
