@@ -59,3 +59,15 @@ def test_argument_errors_are_reported_without_touching_the_device():
     rc = lib.hss_grid_semblance(ctypes.byref(tr), ctypes.byref(mp), 0, ctypes.c_void_p(8), 16, 9,
                                 ctypes.c_void_p(8), None, None, None, None)
     assert rc == -6  # bad variant
+    tr = capi.hss_traces(10, 0, 8, 8, 8, 10, 100, 0.004, 2, 5, 1)
+    rc = lib.hss_grid_semblance(ctypes.byref(tr), ctypes.byref(mp), 0, ctypes.c_void_p(8), 16, 1,
+                                ctypes.c_void_p(8), None, None, None, None)
+    assert rc == -5  # 16-bit samples not on a 4-byte boundary
+    tr = capi.hss_traces(8, 0, 8, 8, 8, 10, (1 << 24) + 1, 0.004, 2, 5, 0)
+    rc = lib.hss_grid_semblance(ctypes.byref(tr), ctypes.byref(mp), 0, ctypes.c_void_p(8), 16, 1,
+                                ctypes.c_void_p(8), None, None, None, None)
+    assert rc == -3  # more samples per trace than the exact window bounds allow
+    tr = capi.hss_traces(8, 0, 8, 8, 8, 10, 100, 0.004, 16, 33, 0)
+    rc = lib.hss_grid_semblance(ctypes.byref(tr), ctypes.byref(mp), 0, ctypes.c_void_p(8), 16, 1,
+                                ctypes.c_void_p(8), None, None, None, None)
+    assert rc == -2  # w above MAX_WINDOW_SIZE - 1
