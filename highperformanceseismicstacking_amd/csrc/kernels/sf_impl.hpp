@@ -20,6 +20,10 @@
 namespace hss {
 
 constexpr int kSfMaxShifts = 64;
+// Each lane's candidate comes from another sample (s - n), so a trace's windows spread
+// over more than one wave segment; 16-bit windows are read per lane (Window::add16),
+// which measured twice as fast as segments on a 2000-midpoint CMP line.
+constexpr bool kSfSegments = false;
 
 struct SfArgs {
     EvalCtx e;
@@ -71,8 +75,8 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
         const float t0n = (float)sc * e.dt;
         float S = 0.0f, stk = 0.0f;
         unsigned rej = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, true>(e, t0n, t0n * t0n, k, (float)n * e.dt, valid,
-                                                   seg_lds[wave], &S, &stk, &rej);
+        evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments>(
+            e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
         if (!valid) {
             S = 0.0f;
             stk = 0.0f;

@@ -161,19 +161,22 @@ def sf_case(model):
 
 @pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
 @pytest.mark.parametrize("variant", [EXACT, FAST])
-def test_stretch_free_against_oracle(model, variant):
+@pytest.mark.parametrize("storage", [F32, F16, BF16])
+def test_stretch_free_against_oracle(model, variant, storage):
+    """EXACT equals the oracle bit for bit (for 16-bit storage: the oracle on the decoded
+    samples); FAST within 1e-5."""
     import torch
     line, b, c, m0, vp, apm, h0 = sf_case(model)
     tau = synth.tau_index(0.02, line.dt_us)
     nvals = np.arange(-10, 11, dtype=np.float32)
-    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
     mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
     res, S, T, _ = capi.stretch_free(dl, mp, model, torch.from_numpy(vp).cuda(), nvals,
                                      variant=variant, full=True, not_used=(model == OCT))
     torch.cuda.synchronize()
     res, S, T = res.cpu().numpy(), S.cpu().numpy(), T.cpu().numpy()
     for g in range(len(b)):
-        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0, apm=apm)
+        og = oracle_gather(dl.decoded, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0, apm=apm)
         r, rS, rT = O.stretch_free_f32(model, og, vp[g].ravel(), nvals, full=True)
         if variant == EXACT:
             assert np.array_equal(_bits(S[g]), _bits(rS)), g
