@@ -63,6 +63,30 @@ def test_de_exact_reproduces_oracle_trajectory(model):
         assert np.array_equal(_bits(res[g]), _bits(r)), g
 
 
+@pytest.mark.parametrize("model", [ZOCRS, OCT])
+@pytest.mark.parametrize("storage", [F16, BF16])
+def test_de_exact_16bit_reproduces_oracle_trajectory(model, storage):
+    """16-bit storage, EXACT: the whole trajectory equals the oracle's on the decoded
+    samples bit for bit (per-lane word reads, two individuals per lane)."""
+    import torch
+    line, b, c, m0, lo, hi, h0, apm, _ = de_case(model)
+    tau = synth.tau_index(0.022, 4000)  # w = 11: the launch shape the benchmarks run
+    NP, G, seed, base = 8, 5, 0x5eed, 3
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
+    res, px, pfx = capi.de_semblance(dl, mp, model, lo, hi, NP, G, seed, stream_base=base,
+                                     variant=EXACT, population=True)
+    torch.cuda.synchronize()
+    res, px, pfx = res.cpu().numpy(), px.cpu().numpy(), pfx.cpu().numpy()
+    for g in range(len(b)):
+        og = oracle_gather(dl.decoded, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0,
+                           apm=apm)
+        r, x, fx = O.de_f32(model, og, lo, hi, NP, G, seed, stream_id=base + g, population=True)
+        assert np.array_equal(_bits(px[g]), _bits(x)), g
+        assert np.array_equal(_bits(pfx[g]), _bits(fx)), g
+        assert np.array_equal(_bits(res[g]), _bits(r)), g
+
+
 def test_de_batch_equals_single_midpoints():
     import torch
     line, b, c, m0, lo, hi, h0, apm, tau = de_case(CMP)

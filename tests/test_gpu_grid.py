@@ -231,13 +231,17 @@ def zocrs_problem(n_mid=3):
     return line, cfg, tau, params, picks, ranges
 
 
-def test_zocrs_grid_exact_bitwise():
+@pytest.mark.parametrize("storage", [F32, BF16])
+def test_zocrs_grid_exact_bitwise(storage):
+    """EXACT equals the oracle bit for bit (16-bit storage: on the decoded samples; the
+    windows then come through the wave segments)."""
     line, cfg, tau, params, picks, ranges = zocrs_problem()
     b = np.array([r[0] for r in ranges])
     c = np.array([r[1] for r in ranges])
-    res, S, T, _, _ = run_gpu(line, b, c, picks, ZOCRS, params, tau, EXACT, apm=cfg["apm"])
+    res, S, T, _, dl = run_gpu(line, b, c, picks, ZOCRS, params, tau, EXACT, storage=storage,
+                               apm=cfg["apm"])
     for g, m0 in enumerate(picks):
-        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0),
+        og = oracle_gather(dl.decoded, line, int(b[g]), int(c[g]), tau, float(m0),
                            apm=cfg["apm"])
         rres, rS, rT, _, _ = O.grid_f32(O.ZOCRS, og, params)
         assert np.array_equal(S[g].view(np.uint32), rS.view(np.uint32)), g
@@ -257,14 +261,15 @@ def oct_problem():
     return line, cfg, tau, params, picks, ranges
 
 
-def test_oct_grid_exact_bitwise():
+@pytest.mark.parametrize("storage", [F32, F16])
+def test_oct_grid_exact_bitwise(storage):
     line, cfg, tau, params, picks, ranges = oct_problem()
     b = np.array([r[0] for r in ranges])
     c = np.array([r[1] for r in ranges])
-    res, S, T, nu, _ = run_gpu(line, b, c, picks, OCT, params, tau, EXACT, h0=cfg["h0"],
-                               apm=cfg["apm"])
+    res, S, T, nu, dl = run_gpu(line, b, c, picks, OCT, params, tau, EXACT, storage=storage,
+                                h0=cfg["h0"], apm=cfg["apm"])
     for g, m0 in enumerate(picks):
-        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0),
+        og = oracle_gather(dl.decoded, line, int(b[g]), int(c[g]), tau, float(m0),
                            h0=cfg["h0"], apm=cfg["apm"])
         rres, rS, rT, _, rnu = O.grid_f32(O.OCT, og, params)
         assert np.array_equal(S[g].view(np.uint32), rS.view(np.uint32)), g
