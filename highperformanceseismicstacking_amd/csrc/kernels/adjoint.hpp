@@ -227,14 +227,34 @@ __device__ __forceinline__ void evaluate_grad(const EvalCtx &e, float t0, float 
             float gi[D];
 #pragma unroll
             for (int m = 0; m < D; ++m) gi[m] = dtd[m] * e.inv_dt;
-            const size_t base = (size_t)ti * (size_t)e.ns + (size_t)first;
-            float y1 = rd(base) * sc;
+            // the window's samples y[0..w]: compiled window sizes read them from the trace's
+            // row (a wave-uniform base and a 32-bit lane offset; 16-bit storage as aligned
+            // words, read16), the runtime-w instantiation sample by sample
+            const size_t trace0 = (size_t)ti * (size_t)e.ns;
+            float y[N + 1];
+            if constexpr (W > 0 && STORAGE == HSS_F32) {
+                const float *p = (const float *)e.samples + trace0 + (unsigned)first;
+#pragma unroll
+                for (int j = 0; j <= N; ++j) y[j] = p[j];
+            } else if constexpr (W > 0) {
+                const size_t row = trace0 - (size_t)e.tau;
+                const unsigned rpar = (unsigned)row & 1u;
+                const uint32_t *roww =
+                    (const uint32_t *)((const uint16_t *)e.samples + (row - rpar));
+                read16<STORAGE, W>(roww, rpar, (unsigned)(first + e.tau), y);
+#pragma unroll
+                for (int j = 0; j <= N; ++j) y[j] = y[j] * sc;
+            } else {
+#pragma unroll
+                for (int j = 0; j <= N; ++j)
+                    y[j] = (j <= e.w) ? rd(trace0 + (size_t)first + j) * sc : 0.0f;
+            }
             float eu = 0.0f;
 #pragma unroll
             for (int j = 0; j < N; ++j) {
                 if (W == 0 && j >= e.w) continue;
-                const float y0 = y1;
-                y1 = rd(base + j + 1) * sc;
+                const float y0 = y[j];
+                const float y1 = y[j + 1];
                 const float d = y1 - y0;
                 const float u = __builtin_fmaf(d, f, y0);
                 A[j] += u;

@@ -19,6 +19,39 @@
 
 namespace hss {
 
+// 16-bit samples read per lane without LDS (compiled window sizes only): the
+// (W+1)/2 + 1 aligned 32-bit words around samples e0 .. e0+W, shifted by one sample when
+// e0 is odd (v_alignbit), widened to float (not scaled).  Words q[0 .. NW-2] cover samples
+// a .. a+W (a = e0 rounded down to even), all inside the window or the sample just before
+// it; the last word is needed only for odd e0, where it holds sample e0+W (and the sample
+// after it, inside the same aligned word); for even e0 it re-reads q[NW-2].  The window's
+// first sample e0 is sample rpar + k of the wave-uniform word row `roww` (the row's first
+// sample rounded down to an even sample, i.e. a 4-byte word of the aligned sample array;
+// rpar = 1 when it was odd), so the loads take a scalar base and a 32-bit lane offset.
+template <int STORAGE, int W>
+__device__ __forceinline__ void read16(const uint32_t *roww, unsigned rpar, unsigned k,
+                                       float (&y)[W + 1]) {
+    constexpr int NW = (W + 1) / 2 + 1;
+    const unsigned e0 = rpar + k;
+    const unsigned p = e0 & 1u;
+    const uint32_t *q = roww + (e0 >> 1);
+    uint32_t D[NW];
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) D[i] = q[i];
+    D[NW - 1] = q[NW - 2 + p];
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) {
+        const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
+        if constexpr (STORAGE == HSS_F16) {
+            y[2 * i] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
+            y[2 * i + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
+        } else {
+            y[2 * i] = __uint_as_float(e << 16);
+            y[2 * i + 1] = __uint_as_float(e & 0xffff0000u);
+        }
+    }
+}
+
 // Kahan step: s + c' = s + c + x with the low part of the sum kept in c (c holds the
 // negated error, as in Kahan's original form).
 __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
@@ -108,50 +141,20 @@ struct Window {
         used++;
     }
 
-    // 16-bit samples read per lane without LDS (compiled window sizes only): the
-    // (W+1)/2 + 1 aligned 32-bit words around samples e0 .. e0+W, shifted by one sample
-    // when e0 is odd (v_alignbit), decoded and scaled.  Words q[0 .. NW-2] cover samples
-    // a .. a+W (a = e0 rounded down to even), all inside the window or the sample just
-    // before it; the last word is needed only for odd e0, where it holds sample e0+W (and
-    // the sample after it, inside the same aligned word); for even e0 it re-reads q[NW-2].
-    // The window's first sample e0 is sample rpar + k of the wave-uniform word row `roww`
-    // (the row's first sample rounded down to an even sample, i.e. a 4-byte word of the
-    // aligned sample array; rpar = 1 when it was odd), so the loads take a scalar base and
-    // a 32-bit lane offset.
+    // 16-bit samples of one window read per lane (read16), then summed.
     template <int STORAGE>
     __device__ __forceinline__ void add16(const uint32_t *roww, unsigned rpar, unsigned k,
                                           float f, float sc) {
         static_assert(W > 0, "add16 needs a compiled window size");
-        constexpr int NW = (W + 1) / 2 + 1;
-        const unsigned e0 = rpar + k;
-        const unsigned p = e0 & 1u;
-        const uint32_t *q = roww + (e0 >> 1);
-        uint32_t D[NW];
-#pragma unroll
-        for (int i = 0; i < NW - 1; ++i) D[i] = q[i];
-        D[NW - 1] = q[NW - 2 + p];
         float y[W + 1];
-#pragma unroll
-        for (int i = 0; i < NW - 1; ++i) {
-            const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
-            if constexpr (STORAGE == HSS_F16) {
-                y[2 * i] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
-                y[2 * i + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
-            } else {
-                y[2 * i] = __uint_as_float(e << 16);
-                y[2 * i + 1] = __uint_as_float(e & 0xffff0000u);
-            }
-        }
+        read16<STORAGE, W>(roww, rpar, k, y);
         if constexpr (VARIANT == HSS_FAST) {
             // the trace's scale factored out of the window: u = sc * (y0 + frac*(y1 - y0))
             // on the stored values, A_j += sc*u and den += sc^2 * sum_j u^2
             float dn = 0.0f;
 #pragma unroll
             for (int j = 0; j < W; ++j) {
-                // fma(y1, 1, -y0) = y1 - y0 exactly; written so for F16 storage, where
-                // both become one mixed-precision FMA that widens the halves itself
-                const float d = __builtin_fmaf(y[j + 1], 1.0f, -y[j]);
-                const float u = __builtin_fmaf(d, f, y[j]);
+                const float u = __builtin_fmaf(y[j + 1] - y[j], f, y[j]);
                 A[j] = __builtin_fmaf(sc, u, A[j]);
                 dn = __builtin_fmaf(u, u, dn);
             }
