@@ -240,42 +240,49 @@ __device__ __forceinline__ OctCand oct_cand(float t0, float h0, const Candidate 
     return o;
 }
 
-// traveltime<HSS_OCT>() from the split terms.
+// traveltime<HSS_OCT>() from the split terms.  The error tests are evaluated as masks,
+// not branches: every lane computes the whole expression (a failed square root or
+// division yields a NaN or inf that is then discarded), and the one branch left skips the
+// time along the trajectory when every lane of the wave dropped the trace at the
+// displaced midpoint.  Radicands of lanes already in error are replaced by 1 so that the
+// square root's small-input path stays off.
 template <bool FAST>
 __device__ __forceinline__ bool oct_traveltime(const OctTrace &tr, const OctCand &k, float m0,
                                                float apm, float *t, bool *reject) {
-    *reject = true;
     // displaced midpoint (oct_displaced_midpoint)
-    float arg = k.tq_sq + k.tq_th * tr.gamma_sq + tr.p16 * k.theta_sq * k.theta_sq;
-    if (arg < 0) return false;
-    arg = k.theta_sq * tr.gamma_sq + k.two_tq + 2.0f * sqrt_of<FAST>(arg);
-    if (arg == 0) return false;
-    const float mh = m0 + k.two_theta * tr.hd / sqrt_of<FAST>(arg);
-    if (fabsf(tr.m - mh) > apm) return false;
-    *reject = false;
+    const float arg1 = k.tq_sq + k.tq_th * tr.gamma_sq + tr.p16 * k.theta_sq * k.theta_sq;
+    const bool bad1 = arg1 < 0;
+    const float arg2 =
+        k.theta_sq * tr.gamma_sq + k.two_tq + 2.0f * sqrt_of<FAST>(bad1 ? 1.0f : arg1);
+    const bool bad2 = bad1 | (arg2 == 0);
+    const float mh = m0 + k.two_theta * tr.hd / sqrt_of<FAST>(bad2 ? 1.0f : arg2);
+    const bool rej = bad2 | (fabsf(tr.m - mh) > apm);
+    *reject = rej;
+    if (rej) return false;
     // time along the trajectory (oct_time)
     const float dmh = (mh - m0) * (mh - m0);
     const float w1 = tr.e1 - dmh;
     const float w2 = tr.e2 - dmh;
-    if (w1 < 0 || w2 < 0) return false;
-    const float u = sqrt_of<FAST>(w1) + sqrt_of<FAST>(w2);
+    const bool bad3 = (w1 < 0) | (w2 < 0);
+    const float u = sqrt_of<FAST>(bad3 ? 1.0f : w1) + sqrt_of<FAST>(bad3 ? 1.0f : w2);
     const float chh = k.c * tr.h * tr.h;
     float th = k.t0;
+    bool bad4 = bad3;
     if (tr.branch == 1) {
-        if (u == 0.0f) return false;
+        bad4 = bad4 | (u == 0.0f);
         const float a = chh + tr.f4 / (u * u) * k.tn0_sq;
-        if (a < 0) return false;
-        th = sqrt_of<FAST>(a);
+        bad4 = bad4 | (a < 0);
+        th = sqrt_of<FAST>(bad4 ? 1.0f : a);
     } else if (tr.branch == 2) {
         const float a = chh + u * u / k.q4 * k.tn0_sq;
-        if (a < 0) return false;
-        th = sqrt_of<FAST>(a);
+        bad4 = bad4 | (a < 0);
+        th = sqrt_of<FAST>(bad4 ? 1.0f : a);
     }
     const float tn_sq = th * th - chh;
-    if (th == 0.0f || k.tn0_sq == 0.0f) return false;
+    const bool bad5 = bad4 | (th == 0.0f) | (k.tn0_sq == 0.0f);
     const float ah = (k.t0 * tn_sq) / (th * k.tn0_sq) * k.slope;
     *t = th + ah * (tr.m - mh);
-    return true;
+    return !bad5;
 }
 
 // Sample index and interpolation weight (COMPUTE_SEMBLANCE, interface.h:27-31):
