@@ -1,0 +1,92 @@
+"""The benchmark workloads at their full size, checked through properties that do not
+depend on size, plus the oracle on a few sampled midpoints.
+
+* C2 (CMP grid, 500 midpoints x 1501 samples x 1024 velocities, w = 11): HSS_EXACT on the
+  whole line equals the FP32 oracle bit for bit on three sampled midpoints; the launch in
+  the XCD-balanced midpoint order equals the natural order bit for bit; the best FAST
+  semblance is within 1e-5 of the best EXACT one everywhere; every semblance is in [0, 1].
+* C3 (ZO-CRS DE, 200 midpoints, NP 32, G 50, BF16): a second run gives the same bits; the
+  reported semblance is the FP32 oracle's at the reported parameters on the decoded
+  samples (within 1e-5, sampled); the Gauss-Newton polish never lowers it (compared in
+  the gradient kernel's arithmetic, which agrees with the DE kernel's within 1e-5).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from highperformanceseismicstacking_amd import CMP, ZOCRS, EXACT, FAST, BF16, capi, synth
+from helpers import oracle_gather, zocrs_bounds, zocrs_window
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def test_c2_full_line():
+    import torch
+    line, cfg = synth.workload("c2")
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    lo, hi, G = cfg["lower"][0], cfg["upper"][0], cfg["granularity"][0]
+    step = float(f32(f32(hi - lo) / f32(G)))
+    b, c, m0 = line.cmp_ranges()
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    params = capi.build_parameter_array(CMP, [lo], [step], [G])
+    nat = capi.DeviceMidpoints(b, c, m0)
+    order = capi.xcd_balanced_order(c)
+    bal = capi.DeviceMidpoints(b[order], c[order], m0[order])
+    exact, _, _, _ = capi.grid_semblance(dl, nat, CMP, params, G, variant=EXACT)
+    fast, _, _, _ = capi.grid_semblance(dl, nat, CMP, params, G, variant=FAST)
+    fast_bal, _, _, _ = capi.grid_semblance(dl, bal, CMP, params, G, variant=FAST)
+    torch.cuda.synchronize()
+    exact, fast, fast_bal = exact.cpu().numpy(), fast.cpu().numpy(), fast_bal.cpu().numpy()
+    # the balanced launch order changes nothing but the order of the results
+    unperm = np.empty_like(fast_bal)
+    unperm[order] = fast_bal
+    assert np.array_equal(unperm.view(np.uint32), fast.view(np.uint32))
+    # semblance bounds (Cauchy-Schwarz) and FAST against EXACT on the whole line
+    S = fast[:, 0, :]
+    assert np.all((S >= 0.0) & (S <= 1.0 + 1e-6))
+    SE = exact[:, 0, :]
+    assert np.all(np.abs(S - SE) <= 1e-5 * np.abs(SE) + 1e-6)
+    # EXACT equals the oracle bit for bit on sampled midpoints
+    pnp = params.cpu().numpy()
+    for g in (0, len(b) // 2, len(b) - 1):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]))
+        rres, _, _, _, _ = O.grid_f32(O.CMP, og, pnp, full=False)
+        assert np.array_equal(exact[g].view(np.uint32), rres.view(np.uint32)), g
+
+
+def test_c3_full_line():
+    import torch
+    line, cfg = synth.workload("c3")
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    lower, upper = zocrs_bounds(cfg["lower"][0], cfg["upper"][0], cfg["ain"], cfg["v0"],
+                                cfg["bpctg"])
+    apm = float(cfg["apm"])
+    _, _, m0s = line.cmp_ranges()
+    ranges = [zocrs_window(line, m, apm) for m in m0s]
+    b = np.array([r[0] for r in ranges])
+    c = np.array([r[1] for r in ranges])
+    NP, G, seed = cfg["population"], cfg["generations"], 20240601
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=BF16)
+    mp = capi.DeviceMidpoints(b, c, m0s, apm=apm)
+    r1, _, _ = capi.de_semblance(dl, mp, ZOCRS, lower, upper, NP, G, seed, variant=FAST)
+    r2, _, _ = capi.de_semblance(dl, mp, ZOCRS, lower, upper, NP, G, seed, variant=FAST)
+    start = capi.semblance_gradient(dl, mp, ZOCRS, r1)
+    pol = capi.polish(dl, mp, ZOCRS, lower, upper, 10, r1)
+    torch.cuda.synchronize()
+    r1, r2, start, pol = (x.cpu().numpy() for x in (r1, r2, start, pol))
+    assert np.array_equal(r1.view(np.uint32), r2.view(np.uint32))
+    valid = r1[:, 0, :] > 0
+    assert valid.mean() > 0.9
+    # the polish compares semblances in the gradient kernel's own arithmetic, which agrees
+    # with the DE kernel's within 1e-5 at the same point
+    S0 = start[:, 0, :][valid]
+    assert np.all(np.abs(S0 - r1[:, 0, :][valid]) <= 1e-5 * S0 + 1e-6)
+    assert np.all(pol[:, 0, :][valid] >= S0)
+    for g in (0, len(b) // 2, len(b) - 1):
+        og = oracle_gather(dl.decoded, line, int(b[g]), int(c[g]), tau, float(m0s[g]), apm=apm)
+        for s in range(0, line.ns, 37):
+            v = f32(r1[g, 2, s])
+            prm = [f32(4.0) / (v * v), r1[g, 3, s], r1[g, 4, s]]
+            S, _, _, _ = O.eval_f32(O.ZOCRS, og, s, prm)
+            assert abs(S - r1[g, 0, s]) <= 1e-5 * abs(S) + 1e-6, (g, s, S, r1[g, 0, s])
