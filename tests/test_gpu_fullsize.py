@@ -9,6 +9,7 @@ depend on size, plus the oracle on a few sampled midpoints.
   reported semblance is the FP32 oracle's at the reported parameters on the decoded
   samples (within 1e-5, sampled); the Gauss-Newton polish never lowers it (compared in
   the gradient kernel's arithmetic, which agrees with the DE kernel's within 1e-5).
+* C4 (OCT DE with device trace selection) and C5 (CMP grid + stretch-free), below.
 """
 import numpy as np
 import pytest
@@ -90,3 +91,79 @@ def test_c3_full_line():
             prm = [f32(4.0) / (v * v), r1[g, 3, s], r1[g, 4, s]]
             S, _, _, _ = O.eval_f32(O.ZOCRS, og, s, prm)
             assert abs(S - r1[g, 0, s]) <= 1e-5 * abs(S) + 1e-6, (g, s, S, r1[g, 0, s])
+
+
+def test_c4_full_line():
+    """C4 (OCT DE, 300 midpoints, NP 48, G 40, FP16 + scale, traces selected on the device):
+    reproducible bit for bit; the reported semblance is the oracle's at the reported
+    parameters on the decoded samples (sampled); the OCT rejection count is that of the
+    selected traces the aperture drops, so it is below the total (trace, evaluation) count."""
+    import torch
+    from highperformanceseismicstacking_amd import OCT, F16
+    line, cfg = synth.workload("c4")
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    lower, upper = cfg["lower"], cfg["upper"]
+    h0, apm = float(cfg.get("h0", 0.0)), float(cfg["apm"])
+    _, _, m0s = line.cmp_ranges()
+    NP, G, seed = cfg["population"], cfg["generations"], 20240601
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=F16)
+    mask = capi.select_traces(dl, OCT, m0s, apm=apm, h0=h0, oct_mode=capi.OCT_DE_START, NP=NP,
+                              lower=lower, upper=upper, seed=seed, stream_base=0)
+    torch.cuda.synchronize()
+    m = mask.cpu().numpy().astype(bool)
+    idx = [np.flatnonzero(m[g]) for g in range(len(m0s))]
+    counts = np.array([len(x) for x in idx])
+    begins = np.r_[0, np.cumsum(counts)[:-1]]
+    mp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm)
+    nu = torch.zeros(mp.n, dtype=torch.int64, device="cuda")
+    r1, _, _ = capi.de_semblance(dl, mp, OCT, lower, upper, NP, G, seed, variant=FAST,
+                                 not_used=nu)
+    r2, _, _ = capi.de_semblance(dl, mp, OCT, lower, upper, NP, G, seed, variant=FAST)
+    torch.cuda.synchronize()
+    r1, r2, nu = r1.cpu().numpy(), r2.cpu().numpy(), nu.cpu().numpy()
+    assert np.array_equal(r1.view(np.uint32), r2.view(np.uint32))
+    total = counts.astype(np.int64) * line.ns * NP * G
+    assert np.all(nu > 0) and np.all(nu < total)
+    for g in (0, len(m0s) // 2, len(m0s) - 1):
+        og = O.Gather(dl.decoded[idx[g]], line.mid[idx[g]], line.hoff[idx[g]], line.dt, tau,
+                      m0=float(m0s[g]), h0=h0, apm=apm)
+        for s in range(0, line.ns, 41):
+            v = f32(r1[g, 2, s])
+            prm = [f32(4.0) / (v * v), r1[g, 3, s]]
+            S, _, _, _ = O.eval_f32(O.OCT, og, s, prm)
+            assert abs(S - r1[g, 0, s]) <= 1e-5 * abs(S) + 1e-6, (g, s, S, r1[g, 0, s])
+
+
+def test_c5_full_line():
+    """C5 (CMP grid, 2000 midpoints x 2001 samples x 2048 velocities, w = 15, then
+    stretch-free with 21 shifts): the grid's best semblance within 1e-5 of the oracle with
+    the same best velocity except ties within 1e-6, and the stretch-free result equal to
+    the oracle's within 1e-5, on sampled midpoints."""
+    import torch
+    line, cfg = synth.workload("c5")
+    tau = synth.tau_index(cfg["tau"], line.dt_us)
+    lo, hi, G = cfg["lower"][0], cfg["upper"][0], cfg["granularity"][0]
+    step = float(f32(f32(hi - lo) / f32(G)))
+    b, c, m0 = line.cmp_ranges()
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(b, c, m0)
+    params = capi.build_parameter_array(CMP, [lo], [step], [G])
+    res, _, _, _ = capi.grid_semblance(dl, mp, CMP, params, G, variant=FAST)
+    vpar = res[:, 2:3, :].contiguous()
+    nvals = np.arange(-10, 11, dtype=np.float32)
+    sf, _, _, _ = capi.stretch_free(dl, mp, CMP, vpar, nvals, variant=FAST)
+    torch.cuda.synchronize()
+    res, sf, vp = res.cpu().numpy(), sf.cpu().numpy(), vpar.cpu().numpy()
+    pnp = params.cpu().numpy()
+    for g in (0, len(b) - 1):
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]))
+        rres, rS, _, rbest, _ = O.grid_f32(O.CMP, og, pnp, full=True)
+        assert np.all(np.abs(res[g, 0] - rres[0]) <= 1e-5 * np.abs(rres[0]) + 1e-6)
+        # the reported velocity is the oracle's best, or ties it within 1e-6
+        vbest = np.float32(2.0) / np.sqrt(pnp[rbest].astype(np.float32))
+        differ = np.flatnonzero(res[g, 2] != vbest)
+        for s in differ:
+            gi = int(np.argmin(np.abs(np.float32(2.0) / np.sqrt(pnp) - res[g, 2, s])))
+            assert abs(float(rS[s, gi]) - float(rS[s, rbest[s]])) <= 1e-6, (g, s)
+        r_sf, _, _ = O.stretch_free_f32(O.CMP, og, vp[g].ravel(), nvals)
+        assert np.all(np.abs(sf[g, 0] - r_sf[0]) <= 1e-5 * np.abs(r_sf[0]) + 1e-6)
