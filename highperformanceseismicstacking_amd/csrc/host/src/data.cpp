@@ -58,7 +58,11 @@ HipDataContainer::HipDataContainer(unsigned int count, std::shared_ptr<DeviceCon
     allocate();
 }
 
-HipDataContainer::~HipDataContainer() { deallocate(); }
+// No exception may leave a destructor: a failing free is ignored here (deallocate()
+// reports it when called directly).
+HipDataContainer::~HipDataContainer() {
+    if (address) (void)hipFree(address);
+}
 
 hipStream_t HipDataContainer::stream() const {
     auto *ctx = dynamic_cast<HipDeviceContext *>(deviceContext.get());
