@@ -94,6 +94,37 @@ def test_linear_search_cmp_exact_matches_oracle(cmp_line):
     assert open(os.path.join(outdir, "traveltime.txt")).read().startswith("==== Model Summary ====\nName = cmp")
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_linear_search_reduced_storage_exact(cmp_line, storage):
+    """--storage f16|bf16 with --variant exact: the program's SU results equal the oracle
+    run on the decoded samples (per-trace power-of-two scale, round to nearest even) bit
+    for bit."""
+    from highperformanceseismicstacking_amd import F16, BF16, capi
+    tmp, path, line = cmp_line
+    lo, hi, G = 1500.0, 4500.0, 40
+    out = tmp / f"storage_{storage}"
+    out.mkdir()
+    outdir, _ = run("single_host_linear_search",
+                    common_args(path, str(out)) + ["--traveltime", "cmp", "--lower-bounds", lo,
+                                                   "--upper-bounds", hi, "--granularity", G,
+                                                   "--variant", "exact", "--storage", storage],
+                    tmp)
+    decoded = capi.encode_samples(line.samples, {"f16": F16, "bf16": BF16}[storage])[2]
+    tau = synth.tau_index(0.02, line.dt_us)
+    params = O.build_parameter_array(O.CMP, [lo], grid_steps([lo], [hi], [G]), [G])
+    res = read_results(outdir, ["coherence", "stack", "velocity"], line.ns)
+    b, c, m0 = line.cmp_ranges()
+    for k, m in enumerate(cdp_list(line)):
+        if m == 0.0:
+            continue
+        got = np.stack([res[n][1][k] for n in ["coherence", "stack", "velocity"]])
+        g = int(np.flatnonzero(m0 == m)[0])
+        og = O.Gather(decoded[b[g]:b[g] + c[g]], line.mid[b[g]:b[g] + c[g]],
+                      line.hoff[b[g]:b[g] + c[g]], line.dt, tau, m0=float(m))
+        rres, _, _, _, _ = O.grid_f32(O.CMP, og, params, full=False)
+        assert np.array_equal(got.view(np.uint32), rres.view(np.uint32)), m
+
+
 def test_batch_size_does_not_change_results(cmp_line):
     tmp, path, line = cmp_line
     args = common_args(path, str(tmp)) + ["--traveltime", "cmp", "--lower-bounds", 1500,
