@@ -187,11 +187,11 @@ def sf_case(model):
 
 
 @pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
-@pytest.mark.parametrize("variant", [EXACT, FAST])
+@pytest.mark.parametrize("variant", [EXACT, FAST, KAHAN])
 @pytest.mark.parametrize("storage", [F32, F16, BF16])
 def test_stretch_free_against_oracle(model, variant, storage):
     """EXACT equals the oracle bit for bit (for 16-bit storage: the oracle on the decoded
-    samples); FAST within 1e-5."""
+    samples); FAST and KAHAN within 1e-5."""
     import torch
     line, b, c, m0, vp, apm, h0 = sf_case(model)
     tau = synth.tau_index(0.02, line.dt_us)
