@@ -19,9 +19,13 @@
 
 namespace hss {
 
-// 16-bit samples read per lane without LDS (compiled window sizes only): the
-// (W+1)/2 + 1 aligned 32-bit words around samples e0 .. e0+W, shifted by one sample when
-// e0 is odd (v_alignbit), widened to float (not scaled).  Words q[0 .. NW-2] cover samples
+// 16-bit samples read per lane without LDS (compiled window sizes only, all odd): the
+// (W+1)/2 + 1 aligned 32-bit words around samples e0 .. e0+W, widened to float (not
+// scaled).  FP16: the words shifted by one sample when e0 is odd (v_alignbit), then
+// converted.  BF16: one v_perm_b32 per sample moves its two bytes into the high half of a
+// zero word (selector byte 0x0c gives 0x00), from either word of the pair, so the shift
+// and the widening are one instruction; sample 2i (2i+1) of the window is bytes {1,0}
+// ({3,2}) of the pair for even e0 and bytes {3,2} ({5,4}) for odd e0.  Words q[0 .. NW-2] cover samples
 // a .. a+W (a = e0 rounded down to even), all inside the window or the sample just before
 // it; the last word is needed only for odd e0, where it holds sample e0+W (and the sample
 // after it, inside the same aligned word); for even e0 it re-reads q[NW-2].  The window's
@@ -31,6 +35,7 @@ namespace hss {
 template <int STORAGE, int W>
 __device__ __forceinline__ void read16(const uint32_t *roww, unsigned rpar, unsigned k,
                                        float (&y)[W + 1]) {
+    static_assert(W & 1, "read16 covers an even number of samples");
     constexpr int NW = (W + 1) / 2 + 1;
     const unsigned e0 = rpar + k;
     const unsigned p = e0 & 1u;
@@ -39,15 +44,20 @@ __device__ __forceinline__ void read16(const uint32_t *roww, unsigned rpar, unsi
 #pragma unroll
     for (int i = 0; i < NW - 1; ++i) D[i] = q[i];
     D[NW - 1] = q[NW - 2 + p];
+    if constexpr (STORAGE == HSS_F16) {
 #pragma unroll
-    for (int i = 0; i < NW - 1; ++i) {
-        const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
-        if constexpr (STORAGE == HSS_F16) {
+        for (int i = 0; i < NW - 1; ++i) {
+            const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
             y[2 * i] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
             y[2 * i + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
-        } else {
-            y[2 * i] = __uint_as_float(e << 16);
-            y[2 * i + 1] = __uint_as_float(e & 0xffff0000u);
+        }
+    } else {
+        const uint32_t sel_even = 0x01000c0cu | (p ? 0x02020000u : 0u);
+        const uint32_t sel_odd = sel_even + 0x02020000u;
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) {
+            y[2 * i] = __uint_as_float(__builtin_amdgcn_perm(D[i + 1], D[i], sel_even));
+            y[2 * i + 1] = __uint_as_float(__builtin_amdgcn_perm(D[i + 1], D[i], sel_odd));
         }
     }
 }
