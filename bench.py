@@ -237,7 +237,7 @@ def run_de(args):
                 e += 1
             begins.append(b)
             counts.append(e - b)
-        mp = capi.DeviceMidpoints(begins, counts, m0s, h0=h0, apm=apm)
+        mp = capi.DeviceMidpoints(begins, counts, m0s, h0=h0, apm=apm, order="balanced")
         counts = np.array(counts)
     else:
         # OCT: the traces some start-population candidate reaches (device selection)
@@ -252,7 +252,8 @@ def run_de(args):
         idx = [np.flatnonzero(m[g]) for g in range(len(m0s))]
         counts = np.array([len(x) for x in idx])
         begins = np.r_[0, np.cumsum(counts)[:-1]]
-        mp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm)
+        mp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm,
+                                  order="balanced")
     np_ = 3 if model == ZOCRS else 2
     nbytes = capi.de_workspace_bytes(mp.n, line.ns, model, NP)
     ws = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device="cuda")
@@ -322,6 +323,7 @@ def run_de(args):
             "de_launch_ms": de_ms,
             "polish_launch_ms": pol_ms,
             "selection_ms": sel_ms,
+            "launch_order": "xcd_balanced",
             "evals_per_step": n_evals,
             "mean_semblance_de": float(resn[:, 0, :][live].mean()),
             "mean_semblance_polished": float(poln[:, 0, :][live].mean()),

@@ -43,6 +43,7 @@ class hss_midpoints(ctypes.Structure):
         ("index", ctypes.c_void_p),
         ("h0", ctypes.c_float),
         ("apm", ctypes.c_float),
+        ("order", ctypes.c_void_p),
     ]
 
 
@@ -181,9 +182,11 @@ class DeviceLine:
 
 
 class DeviceMidpoints:
-    """Midpoint batch: trace ranges (and optional index list) into a DeviceLine."""
+    """Midpoint batch: trace ranges (and optional index list) into a DeviceLine.
+    `order` is the launch order (a permutation of range(n); "balanced" for
+    xcd_balanced_order(count)); it changes the schedule only, never a result."""
 
-    def __init__(self, begin, count, m0, index=None, h0=0.0, apm=0.0, device="cuda"):
+    def __init__(self, begin, count, m0, index=None, h0=0.0, apm=0.0, device="cuda", order=None):
         import torch
         self.begin_h = np.ascontiguousarray(begin, dtype=np.int32)
         self.count_h = np.ascontiguousarray(count, dtype=np.int32)
@@ -197,9 +200,19 @@ class DeviceMidpoints:
         self.m0 = torch.from_numpy(self.m0_h).to(device)
         self.index = (torch.from_numpy(np.ascontiguousarray(index, dtype=np.int32)).to(device)
                       if index is not None else None)
+        if isinstance(order, str):
+            if order != "balanced":
+                raise ValueError(f"unknown order {order!r}")
+            order = xcd_balanced_order(self.count_h)
+        self.order_h = np.ascontiguousarray(order, dtype=np.int32) if order is not None else None
+        if self.order_h is not None and self.order_h.shape != (self.n,):
+            raise ValueError("order must hold one entry per midpoint")
+        self.order = (torch.from_numpy(self.order_h).to(device)
+                      if self.order_h is not None and self.n > 0 else None)
         self.struct = hss_midpoints(
             self.n, self.begin.data_ptr(), self.count.data_ptr(), self.m0.data_ptr(),
-            self.index.data_ptr() if self.index is not None else 0, float(h0), float(apm))
+            self.index.data_ptr() if self.index is not None else 0, float(h0), float(apm),
+            self.order.data_ptr() if self.order is not None else 0)
 
 
 def verify_index_ops(dt, stream=None):

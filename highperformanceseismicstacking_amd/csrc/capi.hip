@@ -170,6 +170,8 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     a.mbeg = mids->begin;
     a.mcnt = mids->count;
     a.mm0 = mids->m0;
+    a.morder = mids->order;
+    a.nmid = mids->n;
     a.params = d_params;
     a.P = P;
     a.ntiles = tiles_of(traces->ns);
@@ -229,6 +231,8 @@ int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int mo
     a.mbeg = mids->begin;
     a.mcnt = mids->count;
     a.mm0 = mids->m0;
+    a.morder = mids->order;
+    a.nmid = mids->n;
     a.NP = NP;
     a.G = G;
     a.gen_begin = gen_begin;
@@ -271,6 +275,8 @@ int hss_stretch_free(const hss_traces *traces, const hss_midpoints *mids, int mo
     a.mbeg = mids->begin;
     a.mcnt = mids->count;
     a.mm0 = mids->m0;
+    a.morder = mids->order;
+    a.nmid = mids->n;
     a.vparams = d_vparams;
     for (int q = 0; q < nN; ++q) a.nvals[q] = nvals[q];
     a.nN = nN;
@@ -305,6 +311,8 @@ static int adjoint_common(const hss_traces *traces, const hss_midpoints *mids, i
     a.mbeg = mids->begin;
     a.mcnt = mids->count;
     a.mm0 = mids->m0;
+    a.morder = mids->order;
+    a.nmid = mids->n;
     a.ntiles = tiles_of(traces->ns);
     a.points = d_points;
     a.out = d_out;

@@ -60,10 +60,14 @@ class MidpointBatch {
 
   private:
     std::vector<float> m0;
-    std::vector<int32_t> beg, cnt, index;
+    std::vector<int32_t> beg, cnt, index, ord;
     bool indexed = false;
-    DeviceBuffer dBeg, dCnt, dM0, dIndex;
+    DeviceBuffer dBeg, dCnt, dM0, dIndex, dOrder;
 };
+
+// Launch order of midpoints with the given work (selected-trace counts) that gives each
+// of the chip's 8 XCDs the same share (hss_midpoints::order).
+std::vector<int32_t> xcd_balanced_order(const std::vector<int32_t> &work);
 
 // Replaces the search result in dRes ([n][(2+np)][ns] on the device) by its Gauss-Newton
 // polish inside the traveltime's parameter bounds when --polish-steps is set.

@@ -1,11 +1,14 @@
 // C API of the host layer (include/hss_host.h).
 #include "hss_host.h"
+#include "hip_backend.hpp"
 #include "semblance_op.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "logger.hpp"
 #include "model.hpp"
@@ -95,6 +98,16 @@ int hss_host_traveltime_bounds(int model, const float *lower, const float *upper
             out_lower[p] = t->getLowerBoundForParameter(p);
             out_upper[p] = t->getUpperBoundForParameter(p);
         }
+    });
+}
+
+int hss_host_xcd_balanced_order(const int32_t *work, int32_t n, int32_t *order) {
+    return guarded([&] {
+        if (n < 0) throw std::invalid_argument("n must be >= 0");
+        if (n > 0 && (!work || !order)) throw std::invalid_argument("work and order are required");
+        const std::vector<int32_t> w(work, work + n);
+        const std::vector<int32_t> o = xcd_balanced_order(w);
+        std::copy(o.begin(), o.end(), order);
     });
 }
 

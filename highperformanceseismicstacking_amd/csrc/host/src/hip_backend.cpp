@@ -156,6 +156,11 @@ void MidpointBatch::upload(hipStream_t stream) {
     dBeg.upload(beg.data(), n * 4, stream);
     dCnt.upload(cnt.data(), n * 4, stream);
     dM0.upload(m0.data(), n * 4, stream);
+    // every launch over the batch takes the midpoints up in the XCD-balanced order; the
+    // results stay in batch order
+    ord = xcd_balanced_order(cnt);
+    dOrder.resize(std::max<size_t>(n, 1) * 4);
+    dOrder.upload(ord.data(), n * 4, stream);
     if (indexed) {
         dIndex.resize(std::max<size_t>(index.size(), 1) * 4);
         dIndex.upload(index.data(), index.size() * 4, stream);
@@ -171,6 +176,7 @@ hss_midpoints MidpointBatch::view(float h0, float apm) const {
     v.index = indexed ? dIndex.as<int32_t>() : nullptr;
     v.h0 = h0;
     v.apm = apm;
+    v.order = m0.size() > 1 ? dOrder.as<int32_t>() : nullptr;
     return v;
 }
 
@@ -203,31 +209,31 @@ static std::vector<std::vector<int32_t>> masks_to_lists(const DeviceBuffer &mask
     return lists;
 }
 
-// Statistics of one midpoint of a batch: its share of the batch's time, in proportion to
-// its (selected traces x evaluations) work.
 // Launch order of a batch's midpoints.  The kernels hand (midpoint, tile) workgroups to
 // the 8 XCDs in contiguous eighths of the launch (logical_block in the kernels), and the
 // folds of a line differ; sorting the midpoints by work and dealing them to 8 bins in
 // snake order (0..7, 7..0, ...) gives every XCD the same number of midpoints with about
-// the same total work, each bin largest first.  Returns the permutation of [0, n).
-static std::vector<size_t> xcd_balanced_order(const std::vector<int> &work) {
+// the same total work, each bin largest first.  Returns a permutation of [0, n).
+std::vector<int32_t> xcd_balanced_order(const std::vector<int32_t> &work) {
     const size_t n = work.size(), bins = 8;
-    std::vector<size_t> idx(n);
-    std::iota(idx.begin(), idx.end(), size_t{0});
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
-    std::vector<std::vector<size_t>> bin(bins);
+    std::vector<int32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return work[a] > work[b]; });
+    std::vector<std::vector<int32_t>> bin(bins);
     const size_t full = n / bins;  // complete rounds; the last partial one fills bins 0.. so
                                    // that the larger bins come first, as the XCD split does
     for (size_t k = 0; k < n; k++) {
         const size_t round = k / bins, pos = k % bins;
         bin[(round & 1) && round < full ? bins - 1 - pos : pos].push_back(idx[k]);
     }
-    std::vector<size_t> order;
+    std::vector<int32_t> order;
     order.reserve(n);
     for (auto &b : bin) order.insert(order.end(), b.begin(), b.end());
     return order;
 }
 
+// Statistics of one midpoint of a batch: its share of the batch's time, in proportion to
+// its (selected traces x evaluations) work.
 static std::array<float, kStatisticCount> batch_statistics(double work, double totalWork,
                                                            double notUsed, double kernelSec,
                                                            double selectionSec, unsigned selected,
@@ -383,20 +389,7 @@ void HipLinearSearchAlgorithm::computeSemblanceAndParametersForMidpoints(
     }
     std::vector<int> counts(m0s.size());
     for (size_t g = 0; g < m0s.size(); g++) counts[g] = batch.count(g);
-    {
-        // one launch for all of them, in an order that balances the XCDs
-        std::vector<int> work(launchM0.size());
-        for (size_t j = 0; j < launchM0.size(); j++) work[j] = counts[slot[j]];
-        const std::vector<size_t> order = xcd_balanced_order(work);
-        std::vector<float> m(launchM0.size());
-        std::vector<size_t> sl(slot.size());
-        for (size_t j = 0; j < order.size(); j++) {
-            m[j] = launchM0[order[j]];
-            sl[j] = slot[order[j]];
-        }
-        launchM0.swap(m);
-        slot.swap(sl);
-    }
+    // one launch for all of them (the batch carries the XCD-balanced launch order)
     selectBatch(launchM0);
     out.assign(m0s.size(), MidpointOutput{});
     std::vector<unsigned long long> nu(launchM0.size(), 0);

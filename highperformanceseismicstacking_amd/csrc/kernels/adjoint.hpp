@@ -308,6 +308,8 @@ struct AdjointArgs {
     const int *mbeg;
     const int *mcnt;
     const float *mm0;
+    const int *morder;  // launch order (launch_midpoint) or NULL
+    int nmid;
     int ntiles;
     const float *points;  // [n][(2+np)][ns]; rows 2.. are the parameters (v space)
     float *out;           // [n][(2+np)][ns]
@@ -337,8 +339,10 @@ __global__ __launch_bounds__(kAdjThreads) void gradient_kernel(AdjointArgs a) {
     constexpr int D = model_np<MODEL>();
     const int lane = threadIdx.x;
     const int blk = logical_block(blockIdx.x, gridDim.x);
-    const int g = blk / a.ntiles;
-    const int tile = blk - g * a.ntiles;
+    const int slot = blk / a.ntiles;
+    const int tile = blk - slot * a.ntiles;
+    const int g = launch_midpoint(a.morder, a.nmid, slot);
+    if (g < 0) return;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;
@@ -373,8 +377,10 @@ __global__ __launch_bounds__(kAdjThreads) void polish_kernel(AdjointArgs a) {
     constexpr int D = model_np<MODEL>();
     const int lane = threadIdx.x;
     const int blk = logical_block(blockIdx.x, gridDim.x);
-    const int g = blk / a.ntiles;
-    const int tile = blk - g * a.ntiles;
+    const int slot = blk / a.ntiles;
+    const int tile = blk - slot * a.ntiles;
+    const int g = launch_midpoint(a.morder, a.nmid, slot);
+    if (g < 0) return;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;

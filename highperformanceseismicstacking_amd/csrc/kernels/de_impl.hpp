@@ -59,6 +59,8 @@ struct DeArgs {
     const int *mbeg;
     const int *mcnt;
     const float *mm0;
+    const int *morder;  // launch order (launch_midpoint) or NULL
+    int nmid;
     int NP, G;
     int gen_begin, gen_end;  // generations run by this launch; the workspace carries the rest
     float lo[3], hi[3];
@@ -135,8 +137,10 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int blk = logical_block(blockIdx.x, gridDim.x);
-    const int g = blk / a.ntiles;
-    const int tile = blk - g * a.ntiles;
+    const int slot = blk / a.ntiles;
+    const int tile = blk - slot * a.ntiles;
+    const int g = launch_midpoint(a.morder, a.nmid, slot);
+    if (g < 0) return;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;
@@ -145,7 +149,7 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
     const uint32_t stream = a.stream_base + (uint32_t)g;
     const int NP = a.NP;
-    float *base = a.ws + (size_t)blk * 2 * NP * REC * kWave;
+    float *base = a.ws + ((size_t)g * a.ntiles + tile) * 2 * NP * REC * kWave;
     auto rec = [&](int buf, int i, int f) -> float * {
         return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
     };

@@ -311,6 +311,17 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
 }
 
 // Context for midpoint g of a batch (fields that do not depend on g come from `base`).
+// The midpoint a workgroup works on: launch slot `slot` (logical block / tiles per
+// midpoint) takes midpoint order[slot] when the caller gives a launch order, else
+// midpoint slot.  An order entry outside [0, n) gives -1: the workgroup has nothing to do.
+// The midpoint, not the slot, indexes everything else (its traces, result rows, workspace
+// rows and DE random stream), so the order changes the schedule and nothing else.
+__device__ __forceinline__ int launch_midpoint(const int *order, int n, int slot) {
+    if (order == nullptr) return slot;
+    const int g = order[slot];
+    return (unsigned)g < (unsigned)n ? g : -1;
+}
+
 __device__ __forceinline__ EvalCtx midpoint_ctx(const EvalCtx &base, const int *mbeg,
                                                 const int *mcnt, const float *mm0, int g) {
     EvalCtx e = base;

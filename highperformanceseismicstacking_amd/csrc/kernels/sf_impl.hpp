@@ -30,6 +30,8 @@ struct SfArgs {
     const int *mbeg;
     const int *mcnt;
     const float *mm0;
+    const int *morder;  // launch order (launch_midpoint) or NULL
+    int nmid;
     const float *vparams;  // [n][np][ns]: velocity, then A,B or slope, per sample
     float nvals[kSfMaxShifts];
     int nN;
@@ -46,8 +48,10 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int blk = logical_block(blockIdx.x, gridDim.x);
-    const int g = blk / a.ntiles;
-    const int tile = blk - g * a.ntiles;
+    const int slot = blk / a.ntiles;
+    const int tile = blk - slot * a.ntiles;
+    const int g = launch_midpoint(a.morder, a.nmid, slot);
+    if (g < 0) return;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
     const bool active = s < ns;

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HSS_ABI_VERSION 1
+#define HSS_ABI_VERSION 2
 
 /* traveltime models (common/include/capability.h:3-7) */
 enum hss_model { HSS_CMP = 0, HSS_ZOCRS = 1, HSS_OCT = 2 };
@@ -83,7 +83,16 @@ typedef struct hss_traces {
 /* The midpoints to process and the traces each one uses.  Midpoint g uses traces
  * index ? index[begin[g] .. begin[g]+count[g]) : [begin[g], begin[g]+count[g]),
  * i.e. the reference's per-midpoint FILT_* compaction (ComputeAlgorithm.cpp:107-153)
- * expressed as ranges into the resident line. */
+ * expressed as ranges into the resident line.
+ *
+ * `order` (no counterpart in the reference; NULL = 0, 1, ..., n-1) is the order in which
+ * a launch takes up the midpoints: launch slot j works on midpoint order[j].  It changes
+ * nothing but the schedule -- midpoint g keeps its result row g, its workspace rows and
+ * its DE random stream (stream_base + g) -- so results are the same bits in any order.
+ * The launch deals slots to the chip's 8 XCDs in 8 contiguous ranges; an order that gives
+ * every range the same total work (largest first) keeps one XCD from finishing last
+ * (hss_host.h: xcd_balanced_order).  It should be a permutation of 0..n-1: slots whose
+ * entry lies outside [0, n) do nothing, and a midpoint listed twice is computed twice. */
 typedef struct hss_midpoints {
     int32_t        n;       /* number of midpoints                                  */
     const int32_t *begin;   /* [n] device                                           */
@@ -92,6 +101,7 @@ typedef struct hss_midpoints {
     const int32_t *index;   /* device or NULL                                       */
     float          h0;      /* OCT reference half offset (Traveltime::getReferenceHalfoffset) */
     float          apm;     /* midpoint aperture (Gather::getApm)                   */
+    const int32_t *order;   /* [n] device or NULL: launch order (see above)         */
 } hss_midpoints;
 
 /* ------------------------------------------------------------------------------ */

@@ -40,6 +40,13 @@ int hss_host_traveltime_bounds(int model, const float *lower, const float *upper
  * opening a device. */
 int hss_host_parse_command_line(int program, int argc, const char **argv);
 
+/* The launch order (hss_midpoints.order) the host layer gives a batch of n midpoints
+ * with the given work (selected-trace counts): sorted by work and dealt to the chip's 8
+ * XCDs in snake order, so that each XCD's eighth of the launch carries the same work,
+ * largest first.  Writes a permutation of 0..n-1 to order[n].  No counterpart in the
+ * reference (it launches one midpoint at a time). */
+int hss_host_xcd_balanced_order(const int32_t *work, int32_t n, int32_t *order);
+
 /* ---- device calls (need a GPU) ------------------------------------------------------
  * SemblanceOp (host/include/semblance_op.hpp), the torch.autograd.Function-shaped C++ op
  * over the adjoint: forward(d_points [n][np][ns]) -> d_S [n][ns], then

@@ -25,6 +25,7 @@ def host():
     L.hss_host_gather_arrays.argtypes = [vp, vp, vp, vp, vp]
     L.hss_host_traveltime_bounds.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int32, f, f, f, vp, vp]
     L.hss_host_parse_command_line.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+    L.hss_host_xcd_balanced_order.argtypes = [vp, ctypes.c_int32, vp]
     return L
 
 
@@ -170,3 +171,27 @@ def test_xcd_balanced_order_is_a_balanced_permutation():
             assert max(sums) - min(sums) <= int(work.max())
         for c in chunks:
             assert np.all(np.diff(c) <= 0)
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 8, 13, 64, 300, 2001])
+def test_xcd_balanced_order(host, n):
+    """The launch order the host layer gives a batch (hss_midpoints.order): a permutation
+    of the midpoints, the same as the Python binding's, largest work first in each of the
+    8 bins, and bins whose total work differs by at most two midpoints' work."""
+    from highperformanceseismicstacking_amd import capi
+    rng = np.random.default_rng(n)
+    work = rng.integers(0, 2500, n).astype(np.int32)
+    work[: n // 4] = work[0] if n else 0  # ties keep the stable order
+    order = np.full(max(n, 1), -7, np.int32)
+    assert host.hss_host_xcd_balanced_order(_p(work), n, _p(order)) == 0
+    order = order[:n]
+    assert sorted(order.tolist()) == list(range(n))
+    assert np.array_equal(order, capi.xcd_balanced_order(work))
+    full, extra = divmod(n, 8)
+    sizes = [full + (b < extra) for b in range(8)]
+    bins = np.split(order, np.cumsum(sizes)[:-1])
+    for b in bins:
+        assert np.all(np.diff(work[b]) <= 0)
+    sums = [int(work[b].sum()) for b in bins]
+    assert max(sums) - min(sums) <= 2 * int(work.max(initial=0))
+    assert host.hss_host_xcd_balanced_order(_p(work), -1, _p(order)) == -1
