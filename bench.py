@@ -250,6 +250,10 @@ def run_de(args):
         sel_ms = e0.elapsed_time(e1)
         m = mask.cpu().numpy().astype(bool)
         idx = [np.flatnonzero(m[g]) for g in range(len(m0s))]
+        if variant != EXACT:
+            # grouped by half offset (as the host layer does): the kernel then computes
+            # the offset-dependent half of the traveltime once per group
+            idx = [x[np.argsort(line.hoff[x], kind="stable")] for x in idx]
         counts = np.array([len(x) for x in idx])
         begins = np.r_[0, np.cumsum(counts)[:-1]]
         mp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm,

@@ -510,7 +510,17 @@ void HipDifferentialEvolutionAlgorithm::selectBatch(const std::vector<float> &m0
             HSS_ASSERT(hss_select_traces(&line.traces(), model, 1, dm0.as<float>(), gather->getApm(),
                                          traveltime->getReferenceHalfoffset(), &oct,
                                          dMask.as<uint8_t>(), stream()));
-            batch.addIndexList(m0s[g], masks_to_lists(dMask, 1, ntr, stream())[0]);
+            std::vector<int32_t> list = masks_to_lists(dMask, 1, ntr, stream())[0];
+            // the non-exact variants sum the traces grouped by half offset: the kernels
+            // then compute the offset-dependent half of the OCT traveltime once per group
+            // (the order changes the sums' rounding only; the exact variant keeps the
+            // reference's order)
+            if (BackendOptions::get().variant != HSS_EXACT) {
+                const std::vector<float> &h = line.halfoffsets();
+                std::stable_sort(list.begin(), list.end(),
+                                 [&](int32_t a, int32_t b) { return h[a] < h[b]; });
+            }
+            batch.addIndexList(m0s[g], list);
         }
     }
     batch.upload(stream());

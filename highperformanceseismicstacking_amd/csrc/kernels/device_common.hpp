@@ -185,8 +185,8 @@ __device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float 
 // run's h0): a wave computes them once per trace instead of once per candidate and lane.
 // OctCand holds the terms of one candidate at one t0.  Every term is the expression the
 // reference spells (oct/common.cuh:7-94), with its operations in the same order as in
-// oct_displaced_midpoint() and oct_time() above, so oct_traveltime() returns the same
-// bits and the same decisions as traveltime<HSS_OCT>().
+// oct_displaced_midpoint() and oct_time() above, so oct_mid(), oct_time_terms() and
+// oct_traveltime_at() give the same bits and the same decisions as traveltime<HSS_OCT>().
 struct OctTrace {
     float h, m;
     float gamma_sq;  // gamma*gamma, gamma = 2*sqrt(h*h + h0*h0)
@@ -240,27 +240,44 @@ __device__ __forceinline__ OctCand oct_cand(float t0, float h0, const Candidate 
     return o;
 }
 
-// traveltime<HSS_OCT>() from the split terms.  The error tests are evaluated as masks,
-// not branches: every lane computes the whole expression (a failed square root or
-// division yields a NaN or inf that is then discarded), and the one branch left skips the
-// time along the trajectory when every lane of the wave dropped the trace at the
-// displaced midpoint.  Radicands of lanes already in error are replaced by 1 so that the
-// square root's small-input path stays off.
+// traveltime<HSS_OCT>() from the split terms, in two halves that depend on the trace's
+// half offset (through OctTrace) and the candidate only: the displaced midpoint
+// (oct_mid) and the time terms (oct_time_terms).  What is left per trace is the aperture
+// test |m - mh| > apm and t = th + ah*(m - mh) (oct_traveltime_at), so traces that share
+// a half offset can share both halves; the values are the same bits either way
+// (oct_traveltime composes them for one trace).  The
+// error tests are evaluated as masks, not branches: every lane computes the whole
+// expression (a failed square root or division yields a NaN or inf that is then
+// discarded).  Radicands of lanes already in error are replaced by 1 so that the square
+// root's small-input path stays off.
+struct OctMid {
+    float mh;
+    bool bad;  // negative first radicand or zero second one (the reference's error codes)
+};
+
 template <bool FAST>
-__device__ __forceinline__ bool oct_traveltime(const OctTrace &tr, const OctCand &k, float m0,
-                                               float apm, float *t, bool *reject) {
+__device__ __forceinline__ OctMid oct_mid(const OctTrace &tr, const OctCand &k, float m0) {
     // displaced midpoint (oct_displaced_midpoint)
     const float arg1 = k.tq_sq + k.tq_th * tr.gamma_sq + tr.p16 * k.theta_sq * k.theta_sq;
     const bool bad1 = arg1 < 0;
     const float arg2 =
         k.theta_sq * tr.gamma_sq + k.two_tq + 2.0f * sqrt_of<FAST>(bad1 ? 1.0f : arg1);
-    const bool bad2 = bad1 | (arg2 == 0);
-    const float mh = m0 + k.two_theta * tr.hd / sqrt_of<FAST>(bad2 ? 1.0f : arg2);
-    const bool rej = bad2 | (fabsf(tr.m - mh) > apm);
-    *reject = rej;
-    if (rej) return false;
+    OctMid d;
+    d.bad = bad1 | (arg2 == 0);
+    d.mh = m0 + k.two_theta * tr.hd / sqrt_of<FAST>(d.bad ? 1.0f : arg2);
+    return d;
+}
+
+struct OctTime {
+    float th, ah;  // t = th + ah*(m - mh)
+    bool bad;
+};
+
+template <bool FAST>
+__device__ __forceinline__ OctTime oct_time_terms(const OctTrace &tr, const OctCand &k, float m0,
+                                                  const OctMid &d) {
     // time along the trajectory (oct_time)
-    const float dmh = (mh - m0) * (mh - m0);
+    const float dmh = (d.mh - m0) * (d.mh - m0);
     const float w1 = tr.e1 - dmh;
     const float w2 = tr.e2 - dmh;
     const bool bad3 = (w1 < 0) | (w2 < 0);
@@ -279,10 +296,35 @@ __device__ __forceinline__ bool oct_traveltime(const OctTrace &tr, const OctCand
         th = sqrt_of<FAST>(bad4 ? 1.0f : a);
     }
     const float tn_sq = th * th - chh;
-    const bool bad5 = bad4 | (th == 0.0f) | (k.tn0_sq == 0.0f);
-    const float ah = (k.t0 * tn_sq) / (th * k.tn0_sq) * k.slope;
-    *t = th + ah * (tr.m - mh);
-    return !bad5;
+    OctTime r;
+    r.bad = bad4 | (th == 0.0f) | (k.tn0_sq == 0.0f);
+    r.th = th;
+    r.ah = (k.t0 * tn_sq) / (th * k.tn0_sq) * k.slope;
+    return r;
+}
+
+// The whole OCT traveltime of one trace: the time terms only where the trace passes the
+// aperture test.  *reject is the aperture decision (notUsedCount); the return value says
+// whether t is a valid traveltime.
+template <bool FAST>
+__device__ __forceinline__ bool oct_traveltime(const OctTrace &tr, const OctCand &k, float m0,
+                                               float apm, float *t, bool *reject) {
+    const OctMid d = oct_mid<FAST>(tr, k, m0);
+    const bool rej = d.bad | (fabsf(tr.m - d.mh) > apm);
+    *reject = rej;
+    if (rej) return false;
+    const OctTime tt = oct_time_terms<FAST>(tr, k, m0, d);
+    *t = tt.th + tt.ah * (tr.m - d.mh);
+    return !tt.bad;
+}
+
+// The per-trace remainder from terms computed earlier for the same half offset.
+__device__ __forceinline__ bool oct_traveltime_at(const OctMid &d, const OctTime &tt, float m,
+                                                  float apm, float *t, bool *reject) {
+    const bool rej = d.bad | (fabsf(m - d.mh) > apm);
+    *reject = rej;
+    *t = tt.th + tt.ah * (m - d.mh);
+    return !rej & !tt.bad;
 }
 
 // Sample index and interpolation weight (COMPUTE_SEMBLANCE, interface.h:27-31):

@@ -123,8 +123,11 @@ __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int li
 // for 16-bit storage, whose per-lane path would read each sample with its own 2-byte
 // load; FP32 windows come in three 16-byte loads per lane, and there the segment's extra
 // instructions cost more than the vector memory path it relieves.
+// OCT_KEEP (OCT only): keep the offset-dependent half of the traveltime across
+// consecutive traces with the same half offset (for lists grouped by offset; in the
+// reference's order it only adds register pressure).
 template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, int CPL,
-          bool SEG = (STORAGE != HSS_F32)>
+          bool SEG = (STORAGE != HSS_F32), bool OCT_KEEP = false>
 __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0sq,
                                            const Candidate (&k)[CPL], const bool (&on)[CPL],
                                            float shift, bool active, float *seg_generic,
@@ -146,6 +149,16 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
 #pragma unroll
         for (int c = 0; c < CPL; ++c) oc[c] = oct_cand(t0, e.h0, k[c]);
     }
+    // OCT_KEEP: the displaced midpoint and the time terms of each candidate depend on the
+    // trace's half offset, not on its midpoint, so they are kept while consecutive traces
+    // share the offset (its bit pattern; a wave-uniform test).  Lists grouped by offset
+    // then cost one aperture test and one multiply-add per trace for most traces; a
+    // candidate's time terms are computed only when some lane keeps the trace for it.
+    OctMid om[CPL];
+    OctTime otm[CPL];
+    uint32_t mid_key = 0;
+    bool mid_valid = false;
+    bool time_valid[CPL];
     // Traces go by in chunks of 64: lane i loads trace j0+i's index, geometry and scale
     // with one coalesced access, and the trace loop broadcasts them with v_readlane, so
     // no memory latency sits inside the per-trace loop.
@@ -182,6 +195,29 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             int kk[CPL];
             float f[CPL];
             bool use[CPL];
+            if constexpr (MODEL == HSS_OCT && OCT_KEEP) {
+                const uint32_t key = __float_as_uint(ot.h);
+                if (!mid_valid || key != mid_key) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) {
+                        om[c] = oct_mid<FASTIDX>(ot, oc[c], e.m0);
+                        time_valid[c] = false;
+                    }
+                    mid_key = key;
+                    mid_valid = true;
+                }
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    if (time_valid[c]) continue;
+                    // exactly the complement of oct_traveltime_at's rejection (a NaN
+                    // displaced midpoint is kept, as in the reference)
+                    const bool need = !om[c].bad & !(fabsf(ot.m - om[c].mh) > e.apm) & on[c];
+                    if (__builtin_amdgcn_ballot_w64(need & active) != 0) {
+                        otm[c] = oct_time_terms<FASTIDX>(ot, oc[c], e.m0, om[c]);
+                        time_valid[c] = true;
+                    }
+                }
+            }
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 kk[c] = 0;
@@ -189,7 +225,11 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                 float t = 0.0f;  // defined for rejected traces too (its index is ignored)
                 bool r;
                 bool ok;
-                if constexpr (MODEL == HSS_OCT)
+                if constexpr (MODEL == HSS_OCT && OCT_KEEP)
+                    // until candidate c's time terms exist for this offset no active lane
+                    // keeps the trace for it, so `ok` and `t` are not used
+                    ok = oct_traveltime_at(om[c], otm[c], ot.m, e.apm, &t, &r);
+                else if constexpr (MODEL == HSS_OCT)
                     ok = oct_traveltime<FASTIDX>(ot, oc[c], e.m0, e.apm, &t, &r);
                 else
                     ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);

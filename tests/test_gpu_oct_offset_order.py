@@ -1,0 +1,118 @@
+"""OCT traces summed in half-offset order (how the host layer and bench.py run the
+non-exact DE variants).
+
+The kernels keep the offset-dependent half of the OCT traveltime (displaced midpoint and
+time terms) while consecutive traces share a half offset.  Index lists grouped by offset
+make that happen on almost every trace; run through the exact variant, the oracle given
+the same list must be matched bit for bit, so the kept terms are the bits a fresh
+computation gives.  The natural (m, h) order, where nothing is kept, is covered by the
+other OCT tests.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from highperformanceseismicstacking_amd import OCT, EXACT, FAST, F32, F16, capi, synth
+from helpers import zocrs_window
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def _case(h0):
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 24)
+    _, _, m0all = line.cmp_ranges()
+    picks = m0all[[5, 11, 17]]
+    apm = 40.0
+    lists = []
+    for m in picks:
+        b, c = zocrs_window(line, m, apm)
+        idx = np.arange(b, b + c)
+        lists.append(idx[np.argsort(line.hoff[idx], kind="stable")])
+    counts = np.array([len(x) for x in lists])
+    begins = np.r_[0, np.cumsum(counts)[:-1]]
+    return line, lists, begins, counts, picks, cfg["lower"], cfg["upper"], h0, apm
+
+
+def _gather(samples, line, idx, tau, m0, h0, apm):
+    return O.Gather(samples[idx], line.mid[idx], line.hoff[idx], line.dt, tau, m0=m0, h0=h0,
+                    apm=apm)
+
+
+@pytest.mark.parametrize("storage", [F32, F16])
+@pytest.mark.parametrize("h0", [0.0, 150.0])
+def test_de_exact_in_offset_order_reproduces_oracle_trajectory(storage, h0):
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(h0)
+    tau = synth.tau_index(0.02, 4000)
+    NP, G, seed, base = 8, 5, 0xC0FFEE, 11
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    res, px, pfx = capi.de_semblance(dl, mp, OCT, lo, hi, NP, G, seed, stream_base=base,
+                                     variant=EXACT, population=True)
+    torch.cuda.synchronize()
+    res, px, pfx = res.cpu().numpy(), px.cpu().numpy(), pfx.cpu().numpy()
+    samples = dl.decoded if storage != F32 else line.samples
+    for g in range(len(m0)):
+        og = _gather(samples, line, lists[g], tau, float(m0[g]), h0, apm)
+        r, x, fx = O.de_f32(OCT, og, lo, hi, NP, G, seed, stream_id=base + g, population=True)
+        assert np.array_equal(_bits(px[g]), _bits(x)), g
+        assert np.array_equal(_bits(pfx[g]), _bits(fx)), g
+        assert np.array_equal(_bits(res[g]), _bits(r)), g
+
+
+@pytest.mark.parametrize("h0", [0.0, 150.0])
+def test_grid_and_stretch_free_exact_in_offset_order_equal_oracle(h0):
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(h0)
+    tau = synth.tau_index(0.02, 4000)
+    Gv, Gs = 24, 7
+    steps = [float(np.float32(np.float32(hi[k] - lo[k]) / np.float32(G)))
+             for k, G in enumerate((Gv, Gs))]
+    params = capi.build_parameter_array(OCT, lo, steps, [Gv, Gs])
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    res, sem, _, nu = capi.grid_semblance(dl, mp, OCT, params, Gv * Gs, variant=EXACT, full=True,
+                                          not_used=True)
+    vpar = res[:, 2:4, :].contiguous()
+    nvals = np.arange(-2, 3, dtype=np.float32)
+    sf, _, _, _ = capi.stretch_free(dl, mp, OCT, vpar, nvals, variant=EXACT)
+    torch.cuda.synchronize()
+    res, sem, sf, vp = res.cpu().numpy(), sem.cpu().numpy(), sf.cpu().numpy(), vpar.cpu().numpy()
+    nu = nu.cpu().numpy()
+    pnp = params.cpu().numpy()
+    for g in range(len(m0)):
+        og = _gather(line.samples, line, lists[g], tau, float(m0[g]), h0, apm)
+        rres, rS, _, _, rnu = O.grid_f32(OCT, og, pnp, full=True)
+        assert np.array_equal(_bits(res[g]), _bits(rres)), g
+        assert np.array_equal(_bits(sem[g]), _bits(rS)), g
+        assert int(nu[g]) == int(rnu), g
+        r_sf, _, _ = O.stretch_free_f32(OCT, og, vp[g].reshape(-1), nvals)
+        assert np.array_equal(_bits(sf[g]), _bits(r_sf)), g
+
+
+def test_de_fast_in_offset_order_reports_semblance_of_its_parameters():
+    """FAST DE over offset-ordered lists (the benchmark's configuration): the reported
+    semblance is the oracle's at the reported parameters, summed in the same order."""
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(0.0)
+    tau = synth.tau_index(0.022, 4000)
+    NP, G, seed = 16, 8, 4242
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=F16)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    res, _, _ = capi.de_semblance(dl, mp, OCT, lo, hi, NP, G, seed, variant=FAST)
+    torch.cuda.synchronize()
+    res = res.cpu().numpy()
+    for g in range(len(m0)):
+        og = _gather(dl.decoded, line, lists[g], tau, float(m0[g]), h0, apm)
+        for s in range(0, line.ns, 17):
+            v = np.float32(res[g, 2, s])
+            if v == 0:
+                continue
+            prm = [np.float32(4.0) / (v * v), res[g, 3, s]]
+            S, _, _, _ = O.eval_f32(OCT, og, s, prm)
+            assert abs(S - res[g, 0, s]) <= 1e-5 * abs(S) + 1e-6, (g, s, S, res[g, 0, s])
