@@ -149,6 +149,25 @@ struct TraceGeom {
 // ZO-CRS square root is the traveltime itself, so a radicand below 2^-96 -- the only
 // inputs where sqrt_rn_big may differ from sqrtf -- gives t < 2^-48 <= dt, t/dt < 1, and
 // no window with tau >= 1 accepts it either way; the small-input branch is not needed.
+// ZO-CRS in two parts, the reference's expression in its own operation order: the
+// midpoint part (t0 + A*dm)^2 + B*dm*dm, then + C*h^2 and the square root.  The first
+// depends on the trace's midpoint only, so a loop over a CMP's traces can keep it.
+__device__ __forceinline__ float zocrs_mid_part(float t0, float dm, const Candidate &k) {
+    const float tmp = t0 + k.p1 * dm;
+    return tmp * tmp + k.p2 * dm * dm;
+}
+
+template <bool FAST>
+__device__ __forceinline__ bool zocrs_time_from(float mid_part, float c_h2, float *t) {
+    const float tmp = mid_part + c_h2;
+    if (tmp >= 0) {
+        if constexpr (FAST) *t = sqrt_rn_big(tmp);
+        else *t = sqrtf(tmp);
+        return true;
+    }
+    return false;
+}
+
 template <int MODEL, bool FAST = false>
 __device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float t0sq, float m0,
                                            float h0, float apm, const Candidate &k, float *t,
@@ -159,15 +178,7 @@ __device__ __forceinline__ bool traveltime(const TraceGeom &tg, float t0, float 
         else *t = sqrtf(t0sq + k.c * tg.h2);
         return true;
     } else if constexpr (MODEL == HSS_ZOCRS) {
-        float dm = tg.m - m0;
-        float tmp = t0 + k.p1 * dm;
-        tmp = tmp * tmp + k.p2 * dm * dm + k.c * tg.h2;
-        if (tmp >= 0) {
-            if constexpr (FAST) *t = sqrt_rn_big(tmp);
-            else *t = sqrtf(tmp);
-            return true;
-        }
-        return false;
+        return zocrs_time_from<FAST>(zocrs_mid_part(t0, tg.m - m0, k), k.c * tg.h2, t);
     } else {
         float mh;
         int err = oct_displaced_midpoint<FAST>(tg.h, h0, t0, m0, k.c, k.p1, &mh);

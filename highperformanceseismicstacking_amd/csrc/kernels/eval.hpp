@@ -154,6 +154,12 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
     // share the offset (its bit pattern; a wave-uniform test).  Lists grouped by offset
     // then cost one aperture test and one multiply-add per trace for most traces; a
     // candidate's time terms are computed only when some lane keeps the trace for it.
+    // ZO-CRS: (t0 + A*dm)^2 + B*dm*dm depends on the trace's midpoint, not its offset; it
+    // is kept while consecutive traces share the midpoint (its bit pattern; a wave-uniform
+    // test), which in the reference's (m, h) order is a whole CMP's run of traces.
+    float zmid[CPL];
+    uint32_t zkey = 0;
+    bool zvalid = false;
     OctMid om[CPL];
     OctTime otm[CPL];
     uint32_t mid_key = 0;
@@ -195,6 +201,16 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             int kk[CPL];
             float f[CPL];
             bool use[CPL];
+            if constexpr (MODEL == HSS_ZOCRS) {
+                const uint32_t key = __float_as_uint(tg.m);
+                if (!zvalid || key != zkey) {
+                    const float dm = tg.m - e.m0;
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) zmid[c] = zocrs_mid_part(t0, dm, k[c]);
+                    zkey = key;
+                    zvalid = true;
+                }
+            }
             if constexpr (MODEL == HSS_OCT && OCT_KEEP) {
                 const uint32_t key = __float_as_uint(ot.h);
                 if (!mid_valid || key != mid_key) {
@@ -231,7 +247,10 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     ok = oct_traveltime_at(om[c], otm[c], ot.m, e.apm, &t, &r);
                 else if constexpr (MODEL == HSS_OCT)
                     ok = oct_traveltime<FASTIDX>(ot, oc[c], e.m0, e.apm, &t, &r);
-                else
+                else if constexpr (MODEL == HSS_ZOCRS) {
+                    r = false;
+                    ok = zocrs_time_from<FASTIDX>(zmid[c], k[c].c * tg.h2, &t);
+                } else
                     ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
                 rej += (r & on[c]) ? 1u : 0u;
                 if constexpr (SHIFT) t = t + shift;
