@@ -54,7 +54,7 @@ constexpr bool kDeSegments = false;
 // OCT: DE keeps the offset-dependent half of the traveltime across consecutive traces
 // with the same half offset (the host layer groups the lists of the non-exact variants by
 // offset).  Any order gives the bits of computing every trace afresh; in the reference's
-// (m, h) order, where nothing is shared, the launch is about 8% slower than a kernel that
+// (m, h) order, where nothing is shared, the launch is about 11% slower than a kernel that
 // does not keep them (measured on the C4 line).
 constexpr bool kDeOctKeep = true;
 constexpr float kDeF = 0.85f;   // F_FAC, common/include/gpu/interface.h:5

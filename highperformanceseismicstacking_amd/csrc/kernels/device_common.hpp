@@ -224,6 +224,27 @@ __device__ __forceinline__ OctTrace oct_trace(float h, float m, float h0) {
     return o;
 }
 
+// Lane jj's value, in a scalar register of the wave.
+__device__ __forceinline__ float bcast(float v, int jj) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), jj));
+}
+
+// Wave-uniform copies of lane jj's offset-dependent terms: those oct_mid reads and those
+// oct_time_terms reads.
+
+__device__ __forceinline__ void bcast_oct_mid_terms(OctTrace &ot, const OctTrace &o, int jj) {
+    ot.gamma_sq = bcast(o.gamma_sq, jj);
+    ot.p16 = bcast(o.p16, jj);
+    ot.hd = bcast(o.hd, jj);
+}
+
+__device__ __forceinline__ void bcast_oct_time_terms(OctTrace &ot, const OctTrace &o, int jj) {
+    ot.e1 = bcast(o.e1, jj);
+    ot.e2 = bcast(o.e2, jj);
+    ot.f4 = bcast(o.f4, jj);
+    ot.branch = __builtin_amdgcn_readlane(o.branch, jj);
+}
+
 struct OctCand {
     float c, slope, t0;
     float theta, theta_sq, two_theta;

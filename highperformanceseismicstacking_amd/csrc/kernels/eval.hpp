@@ -49,10 +49,6 @@ __device__ __forceinline__ TraceGeom load_geom(const float *__restrict__ mid,
     return g;
 }
 
-// Lane jj's value, in a scalar register of the wave.
-__device__ __forceinline__ float bcast(float v, int jj) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), jj));
-}
 
 // Wave segments.  For one trace, the 64 lanes of a wave (consecutive t0 samples, one
 // candidate or nearby ones) read windows that overlap almost entirely: with moveout
@@ -110,8 +106,9 @@ __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int li
 }
 
 // SHIFT: stretch-free evaluation, t = t_n + n*dt (cmp/stretch_free.cu:54-56); `shift` is
-// that n*dt.  Returns S and stack through the pointers and adds the number of model
-// rejections (OCT notUsedCount) to *rejected.
+// that n*dt.  Returns S and stack through the pointers and adds the wave's number of model
+// rejections of active lanes (OCT notUsedCount) to *rejected of its lowest active lane (a
+// caller sums *rejected over the wave's active lanes).
 // The compiled window sizes (W > 0) compute sqrt and t/dt with the verified short
 // sequences (sqrt_rn, div_rn); the runtime-w instantiation (W == 0) uses the library
 // operations and is also what the launchers pick when the check failed for this dt.
@@ -142,7 +139,7 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
     Window<W, VARIANT> acc[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].reset();
-    unsigned rej = 0;
+    unsigned rej = 0;  // the wave's rejections (wave-uniform)
     const int lane = threadIdx.x & (kWave - 1);
     OctCand oc[CPL];
     if constexpr (MODEL == HSS_OCT) {
@@ -186,13 +183,12 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             if constexpr (MODEL == HSS_OCT) {
                 ot.h = bcast(o_l.h, jj);
                 ot.m = bcast(o_l.m, jj);
-                ot.gamma_sq = bcast(o_l.gamma_sq, jj);
-                ot.p16 = bcast(o_l.p16, jj);
-                ot.hd = bcast(o_l.hd, jj);
-                ot.e1 = bcast(o_l.e1, jj);
-                ot.e2 = bcast(o_l.e2, jj);
-                ot.f4 = bcast(o_l.f4, jj);
-                ot.branch = __builtin_amdgcn_readlane(o_l.branch, jj);
+                // OCT_KEEP broadcasts the offset-dependent terms only where it computes
+                // with them
+                if constexpr (!OCT_KEEP) {
+                    bcast_oct_mid_terms(ot, o_l, jj);
+                    bcast_oct_time_terms(ot, o_l, jj);
+                }
             } else {
                 tg.m = bcast(g_l.m, jj);
                 tg.h2 = bcast(g_l.h2, jj);
@@ -214,6 +210,7 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
             if constexpr (MODEL == HSS_OCT && OCT_KEEP) {
                 const uint32_t key = __float_as_uint(ot.h);
                 if (!mid_valid || key != mid_key) {
+                    bcast_oct_mid_terms(ot, o_l, jj);
 #pragma unroll
                     for (int c = 0; c < CPL; ++c) {
                         om[c] = oct_mid<FASTIDX>(ot, oc[c], e.m0);
@@ -229,6 +226,8 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     // displaced midpoint is kept, as in the reference)
                     const bool need = !om[c].bad & !(fabsf(ot.m - om[c].mh) > e.apm) & on[c];
                     if (__builtin_amdgcn_ballot_w64(need & active) != 0) {
+                        // this trace has the kept terms' offset, so its terms are theirs
+                        bcast_oct_time_terms(ot, o_l, jj);
                         otm[c] = oct_time_terms<FASTIDX>(ot, oc[c], e.m0, om[c]);
                         time_valid[c] = true;
                     }
@@ -252,7 +251,8 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                     ok = zocrs_time_from<FASTIDX>(zmid[c], k[c].c * tg.h2, &t);
                 } else
                     ok = traveltime<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k[c], &t, &r);
-                rej += (r & on[c]) ? 1u : 0u;
+                // counted per wave (the callers add the count of active lanes only)
+                rej += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(r & on[c] & active));
                 if constexpr (SHIFT) t = t + shift;
                 // (non-short-circuit: the index of a rejected trace is computed and ignored,
                 // which keeps the lane predicates as masks instead of branches)
@@ -351,7 +351,9 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].reduce(e.w, &S[c], &stack[c]);
-    *rejected += rej;
+    // the wave's count goes to its lowest active lane
+    const unsigned long long act = __builtin_amdgcn_ballot_w64(active);
+    *rejected += (act != 0 && lane == __builtin_ctzll(act)) ? rej : 0u;
 }
 
 // One candidate per lane.
