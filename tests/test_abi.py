@@ -71,3 +71,36 @@ def test_argument_errors_are_reported_without_touching_the_device():
     rc = lib.hss_grid_semblance(ctypes.byref(tr), ctypes.byref(mp), 0, ctypes.c_void_p(8), 16, 1,
                                 ctypes.c_void_p(8), None, None, None, None)
     assert rc == -2  # w above MAX_WINDOW_SIZE - 1
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """The Python binding's structures have the C header's sizes and field offsets (a C
+    program built against include/hss.h prints them)."""
+    import shutil
+    import subprocess
+    from highperformanceseismicstacking_amd import capi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"hss_traces": capi.hss_traces, "hss_midpoints": capi.hss_midpoints,
+               "hss_oct_candidates": capi.hss_oct_candidates}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "hss.h"', 'int main(void) {']
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        name, key, val = line.split()
+        got[(name, key)] = int(val)
+    for name, cls in structs.items():
+        assert ctypes.sizeof(cls) == got[(name, "size")], name
+        for field, _ in cls._fields_:
+            assert getattr(cls, field).offset == got[(name, field)], (name, field)
