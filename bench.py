@@ -344,9 +344,9 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(midpoints shard across GPUs without communication; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (a line would "
+              "shard by midpoint ranges, one process per GPU, with no communication during "
+              "the search)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
