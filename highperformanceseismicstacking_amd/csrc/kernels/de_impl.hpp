@@ -31,9 +31,12 @@ namespace hss {
 //    ZO-CRS sequences better than more resident waves do (OCT -10%, ZO-CRS -6%; three
 //    were slower for ZO-CRS and no better for OCT).
 //  * de_min_occupancy: the register budget, as waves per SIMD the compiler must fit
-//    (OCT 5: 96 VGPRs; ZO-CRS 6: 80; CMP 8: 64), no spills inside the trace loop.
+//    (OCT 6: 80 VGPRs; ZO-CRS 8: 64; CMP 8: 64).  Since the traveltime terms are kept
+//    across traces (offset groups, midpoint runs) the trace loop is short and waits on
+//    memory, and more resident waves pay for the spills of the per-generation code:
+//    OCT 5/6/7/8 waves measured 2435/2334/2356/2479 ms, ZO-CRS 6/7/8 559/557/536 ms.
 //  * de_waves: waves per workgroup, so that whole workgroups fill a CU's wave slots at
-//    that occupancy (OCT 4 x 5 workgroups = 20 per CU; ZO-CRS 8 x 3 = 24; CMP 8 x 4).
+//    that occupancy (OCT 4 x 6 workgroups = 24 per CU; ZO-CRS and CMP 8 x 4 = 32).
 // Kahan windows (twice the accumulators), windows above 11 samples and the runtime-w
 // kernel keep one individual per lane and the compiler's own register budget.
 template <int MODEL>
@@ -45,7 +48,7 @@ constexpr int de_cpl() { return de_tuned<MODEL, W, VARIANT>() && MODEL != HSS_CM
 template <int MODEL, int W, int VARIANT>
 constexpr int de_min_occupancy() {
     if (!de_tuned<MODEL, W, VARIANT>()) return 1;
-    return MODEL == HSS_OCT ? 5 : (MODEL == HSS_ZOCRS ? 6 : 8);
+    return MODEL == HSS_OCT ? 6 : 8;
 }
 constexpr int kDeWavesMax = 8;
 // Every lane carries its own individual, so a trace's windows spread over far more than
