@@ -62,6 +62,29 @@ __device__ __forceinline__ void read16(const uint32_t *roww, unsigned rpar, unsi
     }
 }
 
+// FP16 samples e0 .. e0+W of one window as halves (the words of read16 shifted by one
+// sample for odd e0, not converted): arithmetic on (float)h[j] then reads each half
+// straight from its word (v_fma_mix_f32) instead of converting it first.
+template <int W>
+__device__ __forceinline__ void read16_halves(const uint32_t *roww, unsigned rpar, unsigned k,
+                                              _Float16 (&h)[W + 1]) {
+    static_assert(W & 1, "read16 covers an even number of samples");
+    constexpr int NW = (W + 1) / 2 + 1;
+    const unsigned e0 = rpar + k;
+    const unsigned p = e0 & 1u;
+    const uint32_t *q = roww + (e0 >> 1);
+    uint32_t D[NW];
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) D[i] = q[i];
+    D[NW - 1] = q[NW - 2 + p];
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) {
+        const uint32_t e = __builtin_amdgcn_alignbit(D[i + 1], D[i], 16u * p);
+        h[2 * i] = __builtin_bit_cast(_Float16, (uint16_t)(e & 0xffffu));
+        h[2 * i + 1] = __builtin_bit_cast(_Float16, (uint16_t)(e >> 16));
+    }
+}
+
 // Kahan step: s + c' = s + c + x with the low part of the sum kept in c (c holds the
 // negated error, as in Kahan's original form).
 __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
@@ -156,6 +179,28 @@ struct Window {
     __device__ __forceinline__ void add16(const uint32_t *roww, unsigned rpar, unsigned k,
                                           float f, float sc) {
         static_assert(W > 0, "add16 needs a compiled window size");
+        if constexpr (VARIANT == HSS_FAST && STORAGE == HSS_F16) {
+            // as below, with y1 - y0 formed as fma(y1, one, -y0) = RN(y1 - y0): the same
+            // bits, but an FMA reads both halves straight from the words (v_fma_mix_f32),
+            // where a subtraction needs both converted first.  `one` is 1.0 hidden from the
+            // optimizer (an empty asm statement), which would otherwise turn the FMA back
+            // into a subtraction.
+            _Float16 h[W + 1];
+            read16_halves<W>(roww, rpar, k, h);
+            float one = 1.0f;
+            __asm__("" : "+s"(one));
+            float dn = 0.0f;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const float d = __builtin_fmaf((float)h[j + 1], one, -(float)h[j]);
+                const float u = __builtin_fmaf(d, f, (float)h[j]);
+                A[j] = __builtin_fmaf(sc, u, A[j]);
+                dn = __builtin_fmaf(u, u, dn);
+            }
+            den = __builtin_fmaf(sc * sc, dn, den);
+            used++;
+            return;
+        }
         float y[W + 1];
         read16<STORAGE, W>(roww, rpar, k, y);
         if constexpr (VARIANT == HSS_FAST) {
