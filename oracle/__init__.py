@@ -11,7 +11,6 @@ import numpy as np
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
-REF_DIR = os.path.join(ORACLE_DIR, "_ref")
 
 CMP, ZOCRS, OCT = 0, 1, 2
 NPARAMS = {CMP: 1, ZOCRS: 3, OCT: 2}
@@ -52,9 +51,12 @@ def lib():
         L.orc_stretch_free_f64.argtypes = [i, vp, vp, vp, i, vp, i]
         L.orc_de_f32.argtypes = [i, vp, vp, vp, i, i, ctypes.c_uint64, ctypes.c_uint32,
                                  vp, vp, vp, i]
+        L.orc_de_start_individual.argtypes = [i, vp, vp, vp, vp]
+        L.orc_de_trial.argtypes = [i, i, vp, i, vp, vp, vp, vp, vp]
         L.orc_select_traces.argtypes = [i, vp, vp, i, fl, fl, fl, i, fl, vp, i, i, vp]
         L.orc_oct_selection_samples.argtypes = [fl, fl, fl, fl, i, vp]
         L.orc_eval_grad_f64.argtypes = [i, vp, i, vp, i, vp, vp, vp]
+        L.orc_traveltime_f32.argtypes = [i, vp, i, fl, vp, ctypes.POINTER(fl)]
         _lib = L
     return _lib
 
@@ -174,6 +176,32 @@ def de_f32(model, g, lower, upper, NP, G, seed, stream_id=0, nthreads=8, populat
     return (res, x, fx) if population else res
 
 
+def de_start_individual(lower, upper, u):
+    """x_d = lower_d + u_d * (upper_d - lower_d) for explicit uniforms u [D]."""
+    lo = np.ascontiguousarray(lower, dtype=np.float32)
+    hi = np.ascontiguousarray(upper, dtype=np.float32)
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    out = np.empty(lo.size, np.float32)
+    lib().orc_de_start_individual(lo.size, _p(lo), _p(hi), _p(u), _p(out))
+    return out
+
+
+def de_trial(x, i, lower, upper, u):
+    """(mutant, trial) of individual i of the population x [NP][D] for explicit uniforms
+    u [4D+1]: u[3d..3d+2] donor draws of parameter d, u[3D] the forced crossover index
+    draw, u[3D+1+d] the crossover coin of parameter d."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    NP, D = x.shape
+    lo = np.ascontiguousarray(lower, dtype=np.float32)
+    hi = np.ascontiguousarray(upper, dtype=np.float32)
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    assert u.size == 4 * D + 1 and lo.size == D and hi.size == D
+    mut = np.empty(D, np.float32)
+    tri = np.empty(D, np.float32)
+    lib().orc_de_trial(D, NP, _p(x), int(i), _p(lo), _p(hi), _p(u), _p(mut), _p(tri))
+    return mut, tri
+
+
 def philox(ctr, key):
     c = np.ascontiguousarray(ctr, dtype=np.uint32)
     k = np.ascontiguousarray(key, dtype=np.uint32)
@@ -209,6 +237,15 @@ def oct_time(h, h0, t0, m0, m, mh, c, slope):
     t = ctypes.c_float(0)
     err = lib().orc_oct_time(h, h0, t0, m0, m, mh, c, slope, ctypes.byref(t))
     return err, t.value
+
+
+def traveltime_f32(model, g, i, t0, prm):
+    """(kept, t) of trace i at t0 for the storage-form candidate prm: kept is 0 when the
+    model rejects the trace (ZO-CRS negative radicand, OCT error or aperture)."""
+    prm = np.ascontiguousarray(prm, dtype=np.float32)
+    t = ctypes.c_float(0)
+    kept = lib().orc_traveltime_f32(model, g.ref, int(i), float(t0), _p(prm), ctypes.byref(t))
+    return kept, t.value
 
 
 def eval_grad_f64(model, g, s, pv, f32_index=False):

@@ -560,6 +560,41 @@ static void orc_de_eval(int model, const orc_gather *g, int s, const float *xv,
     orc_eval_f32(model, g, s, 0, prm, S, stk, NULL, NULL);
 }
 
+/* One start individual from D uniforms: x_d = lower_d + U_d * (upper_d - lower_d)
+ * (common/differential_evolution.cu:19-40). */
+void orc_de_start_individual(int D, const float *lower, const float *upper, const float *u,
+                             float *xi)
+{
+    for (int d = 0; d < D; d++) {
+        xi[d] = lower[d] + u[d] * (upper[d] - lower[d]);
+    }
+}
+
+/* Mutant and trial of individual i of population x [NP][D] from explicit uniforms:
+ * u[3d + 0..2] are the donor draws of parameter d (mutatePopulations,
+ * common/differential_evolution.cu:43-86: r = floor(U * (NP - 1)), v = x_r1 + F (x_r2 - x_r3),
+ * fminf with the upper bound, then fmaxf with the lower), u[3D] the forced index
+ * l = floor(U * (D - 1)) and u[3D + 1 + d] the coin of parameter d (crossoverPopulations,
+ * :88-117: the trial keeps x_d when coin > CR and d != l, else takes v_d). */
+void orc_de_trial(int D, int NP, const float *x, int i, const float *lower, const float *upper,
+                  const float *u, float *mutant, float *trial)
+{
+    for (int d = 0; d < D; d++) {
+        unsigned r1 = (unsigned)(u[3 * d] * (float)(NP - 1));
+        unsigned r2 = (unsigned)(u[3 * d + 1] * (float)(NP - 1));
+        unsigned r3 = (unsigned)(u[3 * d + 2] * (float)(NP - 1));
+        float nv = x[r1 * D + d] + ORC_F_FAC * (x[r2 * D + d] - x[r3 * D + d]);
+        nv = fminf(nv, upper[d]);
+        nv = fmaxf(nv, lower[d]);
+        mutant[d] = nv;
+    }
+    unsigned l = (unsigned)(u[3 * D] * (float)(D - 1));
+    for (int d = 0; d < D; d++) {
+        float r = u[3 * D + 1 + d];
+        trial[d] = (r > ORC_CR && l != (unsigned)d) ? x[i * D + d] : mutant[d];
+    }
+}
+
 void orc_de_f32(int model, const orc_gather *g, const float *lower, const float *upper,
                 int NP, int G, uint64_t seed, uint32_t stream_id, float *result,
                 float *x_out, float *fx_out, int nthreads)
@@ -580,30 +615,17 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
             float un[16];
             for (int i = 0; i < NP; i++) {
                 orc_de_uniforms(seed, stream_id, s, i, 0, D, un);
-                for (int d = 0; d < D; d++) {
-                    x[i * D + d] = lower[d] + un[d] * (upper[d] - lower[d]);
-                }
+                orc_de_start_individual(D, lower, upper, un, x + i * D);
             }
             for (int i = 0; i < NP; i++) {
                 orc_de_eval(model, g, s, x + i * D, &fx[2 * i], &fx[2 * i + 1]);
             }
             for (int gen = 1; gen < G; gen++) {
+                /* synchronous generation: every trial is formed from the population of
+                 * the previous generation before any individual is replaced */
                 for (int i = 0; i < NP; i++) {
                     orc_de_uniforms(seed, stream_id, s, i, gen, 4 * D + 1, un);
-                    for (int d = 0; d < D; d++) {
-                        unsigned r1 = (unsigned)(un[3 * d] * (float)(NP - 1));
-                        unsigned r2 = (unsigned)(un[3 * d + 1] * (float)(NP - 1));
-                        unsigned r3 = (unsigned)(un[3 * d + 2] * (float)(NP - 1));
-                        float nv = x[r1 * D + d] + ORC_F_FAC * (x[r2 * D + d] - x[r3 * D + d]);
-                        nv = fminf(nv, upper[d]);
-                        nv = fmaxf(nv, lower[d]);
-                        vv[i * D + d] = nv;
-                    }
-                    unsigned l = (unsigned)(un[3 * D] * (float)(D - 1));
-                    for (int d = 0; d < D; d++) {
-                        float r = un[3 * D + 1 + d];
-                        uu[i * D + d] = (r > ORC_CR && l != (unsigned)d) ? x[i * D + d] : vv[i * D + d];
-                    }
+                    orc_de_trial(D, NP, x, i, lower, upper, un, vv + i * D, uu + i * D);
                 }
                 for (int i = 0; i < NP; i++) {
                     orc_de_eval(model, g, s, uu + i * D, &fu[2 * i], &fu[2 * i + 1]);

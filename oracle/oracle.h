@@ -113,6 +113,16 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
                 int NP, int G, uint64_t seed, uint32_t stream_id, float *result,
                 float *x_out, float *fx_out, int nthreads);
 
+/* The DE operators on explicit uniforms (the pieces orc_de_f32 is made of):
+ * orc_de_start_individual: x_d = lower_d + u[d] * (upper_d - lower_d);
+ * orc_de_trial: the mutant and trial of individual i of x [NP][D], with u[3d+0..2] the
+ * donor draws of parameter d, u[3D] the forced crossover index draw and u[3D+1+d] the
+ * crossover coin of parameter d. */
+void orc_de_start_individual(int D, const float *lower, const float *upper, const float *u,
+                             float *xi);
+void orc_de_trial(int D, int NP, const float *x, int i, const float *lower, const float *upper,
+                  const float *u, float *mutant, float *trial);
+
 /* FP64 semblance, stack and dS/d(parameters) at one sample s for one parameter point
  * pv in v space (v; v,A,B; v,slope) (oracle_grad.c).  grad[np] may be NULL.
  *   f32_index = 0: everything in double, including the traveltime, t/dt and the sample

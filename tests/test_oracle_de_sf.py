@@ -1,0 +1,332 @@
+"""Pins the oracle's differential-evolution and stretch-free functions without relying on
+their own formulas (the GPU kernels are compared with these functions bit for bit, so a
+shared mistake would otherwise go unseen).
+
+Differential evolution (common/differential_evolution.cu:19-155, driven by
+DifferentialEvolutionAlgorithm.cpp:21-81):
+  * the operators on explicit uniforms, against values worked out by hand: donor index
+    floor(U * (NP - 1)) (so U = 1 is the only way to reach the last individual),
+    F = 0.85, the clamp to [lower, upper], the forced crossover index floor(U * (D - 1))
+    (parameter D - 1 is forced only by U = 1), coins kept on strict '> 0.5';
+  * D = 1: the trial is the mutant whatever the coin;
+  * whole runs with closed forms: with NP = 2 every donor is individual 0, so the mutant
+    is x0 and after one generation individual 1 holds x0 exactly when S(x0) > S(x1);
+    with NP = 3 every mutant lies in {x_a + 0.85 (x_b - x_c) : a, b, c in {0, 1}} clamped,
+    and a replaced individual holds such a mutant with a larger semblance.
+
+Stretch-free (cmp|zocrs|oct/stretch_free.cu, common/stretch_free.cu:3-35):
+  * one shift n = 0 equals the pinned single evaluation at v(s) bit for bit;
+  * a gather delayed by n0 whole samples, re-stacked with shift n0, gives the semblance of
+    the undelayed gather at sample s - n0 (within the rounding of t + n0 dt);
+  * several shifts give the first strict maximum of the single-shift results.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from highperformanceseismicstacking_amd import synth
+
+f32 = np.float32
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+# ---------------------------------------------------------------- DE operators by hand
+X4 = np.array([[1000.0, 1e-4, 1e-7],
+               [2000.0, 2e-4, 2e-7],
+               [3000.0, -3e-4, 3e-7],
+               [4000.0, 4e-4, 4e-7]], np.float32)
+LO3 = np.array([500.0, -1e-3, -1e-6], np.float32)
+HI3 = np.array([5000.0, 1e-3, 1e-6], np.float32)
+
+# with NP = 4 a draw U picks individual floor(3 U): 0.1 -> 0, 0.4 -> 1, 0.7 -> 2, 1.0 -> 3
+PICK = {0: 0.1, 1: 0.4, 2: 0.7, 3: 1.0}
+
+
+def _u(donors, l_draw, coins):
+    u = []
+    for (a, b, c) in donors:
+        u += [PICK[a], PICK[b], PICK[c]]
+    return np.array(u + [l_draw] + list(coins), np.float32)
+
+
+def test_de_mutant_values_by_hand():
+    # parameter 0: x1 + 0.85 (x2 - x0) = 2000 + 0.85 * 2000 = 3700
+    # parameter 1: x3 + 0.85 (x0 - x2) = 4e-4 + 0.85 * 4e-4 = 7.4e-4 (U = 1 reaches x3)
+    # parameter 2: x0 + 0.85 (x3 - x0) = 1e-7 + 0.85 * 3e-7 = 3.55e-7
+    u = _u([(1, 2, 0), (3, 0, 2), (0, 3, 0)], 0.3, [0.1, 0.1, 0.1])
+    mut, tri = O.de_trial(X4, 2, LO3, HI3, u)
+    assert mut[0] == f32(3700.0)
+    assert abs(float(mut[1]) - 7.4e-4) <= 2e-10      # 3 ulps
+    assert abs(float(mut[2]) - 3.55e-7) <= 1e-13     # 3 ulps
+    # every coin is below 0.5, so the trial is the mutant
+    assert np.array_equal(_bits(tri), _bits(mut))
+
+
+def test_de_mutant_is_clamped_to_the_bounds():
+    # x3 + 0.85 (x3 - x0) = 6550 > 5000;  x0 + 0.85 (x0 - x3) = -1550 < 500
+    u = _u([(3, 3, 0), (0, 0, 0), (0, 0, 0)], 0.3, [0.1, 0.1, 0.1])
+    mut, _ = O.de_trial(X4, 0, LO3, HI3, u)
+    assert mut[0] == f32(5000.0)
+    u = _u([(0, 0, 3), (0, 0, 0), (0, 0, 0)], 0.3, [0.1, 0.1, 0.1])
+    mut, _ = O.de_trial(X4, 0, LO3, HI3, u)
+    assert mut[0] == f32(500.0)
+    # equal donors give the donor itself
+    assert mut[1] == X4[0, 1] and mut[2] == X4[0, 2]
+
+
+def test_de_donor_index_uses_np_minus_one():
+    # U just below 1 must not reach the last individual (floor(0.999 * 3) = 2), U = 1 does
+    x = X4.copy()
+    u = np.array([0.999, 0.1, 0.1] + [0.1] * 6 + [0.3, 0.1, 0.1, 0.1], np.float32)
+    mut, _ = O.de_trial(x, 0, LO3, HI3, u)
+    assert mut[0] == x[2, 0]
+    u[0] = 1.0
+    mut, _ = O.de_trial(x, 0, LO3, HI3, u)
+    assert mut[0] == x[3, 0]
+
+
+@pytest.mark.parametrize("l_draw,coins,want_from_mutant", [
+    (0.3, [0.9, 0.9, 0.9], [True, False, False]),    # l = floor(0.6) = 0 forced
+    (0.6, [0.9, 0.9, 0.9], [False, True, False]),    # l = floor(1.2) = 1 forced
+    (0.999, [0.9, 0.9, 0.9], [False, True, False]),  # floor(1.998) = 1: D - 1 not reached
+    (1.0, [0.9, 0.9, 0.9], [False, False, True]),    # U = 1 forces the last parameter
+    (0.3, [0.9, 0.2, 0.9], [True, True, False]),     # a coin below CR takes the mutant
+    (0.6, [0.5, 0.9, 0.51], [True, True, False]),    # a coin equal to CR is not '> CR'
+])
+def test_de_crossover_forced_index_and_coins(l_draw, coins, want_from_mutant):
+    u = _u([(1, 2, 0), (3, 0, 2), (0, 3, 0)], l_draw, coins)
+    i = 2
+    mut, tri = O.de_trial(X4, i, LO3, HI3, u)
+    for d in range(3):
+        want = mut[d] if want_from_mutant[d] else X4[i, d]
+        assert _bits(tri[d]) == _bits(want), d
+
+
+def test_de_single_parameter_trial_is_the_mutant():
+    # D = 1: l = floor(U * 0) = 0 is always the one parameter
+    x = np.array([[1500.0], [2500.0], [3500.0]], np.float32)
+    for coin in (0.99, 0.51, 1.0):
+        u = np.array([0.7, 0.7, 0.1, 0.8, coin], np.float32)   # donors 1, 1, 0
+        mut, tri = O.de_trial(x, 0, [1000.0], [4000.0], u)
+        assert mut[0] == f32(2500.0) + f32(0.85) * (f32(2500.0) - f32(1500.0))
+        assert tri[0] == mut[0]
+
+
+def test_de_start_individual():
+    lo, hi = np.array([1500.0, -5e-4], np.float32), np.array([4500.0, 5e-4], np.float32)
+    x = O.de_start_individual(lo, hi, [0.5, 1.0])
+    assert x[0] == f32(3000.0) and x[1] == f32(5e-4)
+
+
+# ---------------------------------------------------------------- DE runs, closed forms
+def _cmp_gather(seed=13, ns=201):
+    line = synth.make_line(1, 30, 2500.0, ns, 4000, seed=seed)
+    return O.Gather(line.samples, line.mid, line.hoff, line.dt, 4, m0=float(line.mid[0]))
+
+
+def _zocrs_gather(seed=21, ns=161):
+    line = synth.make_line(9, 12, 1500.0, ns, 4000, seed=seed, kind="zocrs",
+                           zocrs=((-8e-4, 8e-4), (-8e-7, 8e-7), (2e-7, 1.4e-6)))
+    m0 = float(line.mid[len(line.mid) // 2])
+    return O.Gather(line.samples, line.mid, line.hoff, line.dt, 4, m0=m0, apm=60.0)
+
+
+def _S_of(model, g, s, xv):
+    prm = [f32(4.0) / (f32(xv[0]) * f32(xv[0]))] + [f32(v) for v in xv[1:]]
+    return O.eval_f32(model, g, s, np.array(prm, np.float32))[:2]
+
+
+def test_de_start_population_follows_the_philox_stream():
+    g = _cmp_gather()
+    seed, stream = 0x1234_5678_9abc, 7
+    _, x, fx = O.de_f32(O.CMP, g, [1500.0], [4500.0], 5, 1, seed, stream, population=True)
+    key = [seed & 0xffffffff, seed >> 32]
+    for s in (0, 17, 200):
+        for i in range(5):
+            r = O.philox([s, stream, i, 0], key)
+            u = f32((float(r[0] >> 8) + 1.0) * 2.0 ** -24)
+            assert x[s, i, 0] == f32(1500.0) + u * (f32(4500.0) - f32(1500.0))
+            S, T = _S_of(O.CMP, g, s, x[s, i])
+            assert _bits(fx[s, i, 0]) == _bits(S) and _bits(fx[s, i, 1]) == _bits(T)
+
+
+def test_de_two_individuals_closed_form_cmp():
+    g = _cmp_gather()
+    lo, hi = [1500.0], [4500.0]
+    _, x1, f1 = O.de_f32(O.CMP, g, lo, hi, 2, 1, 99, 3, population=True)
+    res, x2, f2 = O.de_f32(O.CMP, g, lo, hi, 2, 2, 99, 3, population=True)
+    moved = 0
+    for s in range(g.ns):
+        x0, xb = x1[s, 0, 0], x1[s, 1, 0]
+        S0, Sb = f1[s, 0, 0], f1[s, 1, 0]
+        assert x2[s, 0, 0] == x0                      # trial of 0 is x0 itself: no change
+        want = x0 if S0 > Sb else xb
+        assert x2[s, 1, 0] == want, s
+        moved += want != xb
+        best = 0 if f2[s, 0, 0] >= f2[s, 1, 0] or np.isnan(f2[s, 1, 0]) else 1
+        if not np.isnan(f2[s, best, 0]):
+            assert res[2, s] == x2[s, best, 0]
+    assert moved > 10
+
+
+def test_de_two_individuals_closed_form_zocrs():
+    # NP = 2, D = 3: the mutant is x0; the trial of individual 1 takes x0 in the forced
+    # parameter (0 or 1, never 2 short of U = 1) and where the coin is <= 0.5, x1 elsewhere
+    g = _zocrs_gather()
+    lo, hi = [1600.0, -1e-3, -1e-6], [4500.0, 1e-3, 1e-6]
+    _, x1, f1 = O.de_f32(O.ZOCRS, g, lo, hi, 2, 1, 5, 0, population=True)
+    _, x2, f2 = O.de_f32(O.ZOCRS, g, lo, hi, 2, 2, 5, 0, population=True)
+    moved = 0
+    for s in range(g.ns):
+        a, b = x1[s, 0], x1[s, 1]
+        assert np.array_equal(_bits(x2[s, 0]), _bits(a))
+        nb = x2[s, 1]
+        if np.array_equal(_bits(nb), _bits(b)):
+            continue
+        moved += 1
+        from_a = [nb[d] == a[d] and a[d] != b[d] for d in range(3)]
+        assert all(nb[d] == a[d] or nb[d] == b[d] for d in range(3)), s
+        assert from_a[0] or from_a[1] or a[0] == b[0] or a[1] == b[1], s
+        S, T = _S_of(O.ZOCRS, g, s, nb)
+        assert _bits(f2[s, 1, 0]) == _bits(S) and S > f1[s, 1, 0]
+    assert moved > 10
+
+
+def test_de_three_individuals_mutants_in_the_closed_set():
+    g = _cmp_gather(seed=17)
+    lo, hi = f32(1500.0), f32(4500.0)
+    _, x1, f1 = O.de_f32(O.CMP, g, [lo], [hi], 3, 1, 42, 1, population=True)
+    _, x2, f2 = O.de_f32(O.CMP, g, [lo], [hi], 3, 2, 42, 1, population=True)
+    F = f32(0.85)
+    replaced = 0
+    for s in range(g.ns):
+        xs = x1[s, :, 0]
+        cand = set()
+        for a in (0, 1):
+            for b in (0, 1):
+                for c in (0, 1):
+                    m = f32(xs[a] + F * f32(xs[b] - xs[c]))
+                    cand.add(float(max(min(m, hi), lo)))
+        for i in range(3):
+            if x2[s, i, 0] == xs[i]:
+                continue
+            replaced += 1
+            assert float(x2[s, i, 0]) in cand, (s, i)
+            S, _ = _S_of(O.CMP, g, s, x2[s, i])
+            assert S > f1[s, i, 0] and _bits(S) == _bits(f2[s, i, 0])
+    assert replaced > 20
+
+
+# ---------------------------------------------------------------- stretch-free
+def _gathers():
+    cmp = _cmp_gather(seed=31, ns=241)
+    zo = _zocrs_gather(seed=32, ns=241)
+    line = synth.make_line(11, 10, 2400.0, 241, 4000, seed=33, kind="oct",
+                           oct_slope=(-3e-4, 3e-4), offset_min=200.0)
+    m0 = float(line.mid[len(line.mid) // 2])
+    oct_ = O.Gather(line.samples, line.mid, line.hoff, line.dt, 4, m0=m0, h0=300.0, apm=80.0)
+    return {O.CMP: cmp, O.ZOCRS: zo, O.OCT: oct_}
+
+
+def _vparams(model, ns, rng):
+    # velocities that jump from sample to sample, so reading the wrong sample shows
+    rows = [rng.uniform(1600.0, 4400.0, ns)]
+    if model == O.ZOCRS:
+        rows += [rng.uniform(-6e-4, 6e-4, ns), rng.uniform(-6e-7, 6e-7, ns)]
+    elif model == O.OCT:
+        rows += [rng.uniform(-2e-4, 2e-4, ns)]
+    return np.array(rows, np.float32)
+
+
+def _storage(model, vp, s):
+    v = vp[0, s]
+    return np.array([f32(4.0) / (v * v)] + list(vp[1:, s]), np.float32)
+
+
+@pytest.mark.parametrize("model", [O.CMP, O.ZOCRS, O.OCT])
+def test_stretch_free_zero_shift_is_the_single_evaluation(model):
+    g = _gathers()[model]
+    vp = _vparams(model, g.ns, np.random.default_rng(40 + model))
+    res, sembl, stack = O.stretch_free_f32(model, g, vp, [0.0], full=True)
+    for s in range(g.ns):
+        S, T, _, _ = O.eval_f32(model, g, s, _storage(model, vp, s))
+        assert _bits(sembl[s, 0]) == _bits(S) and _bits(stack[s, 0]) == _bits(T), s
+        if S > -1:      # NaN never wins: S and stack stay -1 and 0, n stays 0
+            assert _bits(res[0, s]) == _bits(S) and res[2, s] == 0.0
+
+
+def _delayed(g, n0):
+    """The gather delayed by n0 whole samples: n0 > 0 prepends n0 zero samples,
+    n0 < 0 drops the first |n0| samples."""
+    if n0 >= 0:
+        smp = np.concatenate([np.zeros((g.samples.shape[0], n0), np.float32), g.samples], axis=1)
+    else:
+        smp = g.samples[:, -n0:]
+    s = g.s
+    return O.Gather(smp, g.mid, g.hoff, s.dt, s.tau, m0=s.m0, h0=s.h0, apm=s.apm, hoff2=g.hoff2)
+
+
+def _interior(model, g, sn, prm, margin):
+    """True when every trace's window lies at least `margin` samples inside the trace
+    (then a delayed copy includes and excludes the same traces)."""
+    tau = g.s.tau
+    for i in range(g.samples.shape[0]):
+        ok, t = O.traveltime_f32(model, g, i, f32(f32(sn) * f32(g.s.dt)), prm)
+        if not ok:
+            continue
+        k = t / g.s.dt
+        if not (tau + margin <= k <= g.ns - tau - 2 - margin):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("model", [O.CMP, O.ZOCRS, O.OCT])
+@pytest.mark.parametrize("n0", [3, -2, 7])
+def test_stretch_free_delayed_gather_gives_the_undelayed_semblance(model, n0):
+    g = _gathers()[model]
+    vp = _vparams(model, g.ns, np.random.default_rng(50 + model))
+    gd = _delayed(g, n0)
+    nsd = gd.ns
+    # vparams of the delayed gather: sample s - n0 of the delayed gather reads the
+    # parameters of sample s - n0 of the original
+    vpd = np.zeros((vp.shape[0], nsd), np.float32)
+    m = min(nsd, g.ns)
+    vpd[:, :m] = vp[:, :m]
+    res, sembl, stack = O.stretch_free_f32(model, gd, vpd, [float(n0)], full=True)
+    checked = 0
+    for s in range(nsd):
+        sn = s - n0
+        if not (0 <= sn < min(g.ns, nsd)):
+            continue
+        prm = _storage(model, vp, sn)
+        if not _interior(model, g, sn, prm, 2):
+            continue
+        S, T, used, _ = O.eval_f64(model, g, sn, prm)
+        if used == 0 or not np.isfinite(S):
+            continue
+        assert abs(float(sembl[s, 0]) - S) <= 2e-3 * max(S, 1e-2), (s, sembl[s, 0], S)
+        assert abs(float(stack[s, 0]) - T) <= 2e-3 * max(abs(T), 1e-2), (s, stack[s, 0], T)
+        checked += 1
+    assert checked >= 30
+
+
+@pytest.mark.parametrize("model", [O.CMP, O.ZOCRS, O.OCT])
+def test_stretch_free_picks_the_first_strict_maximum_over_shifts(model):
+    g = _gathers()[model]
+    vp = _vparams(model, g.ns, np.random.default_rng(60 + model))
+    nvals = [-4.0, 0.0, 2.0, 2.0, 5.0]
+    res, _, _ = O.stretch_free_f32(model, g, vp, nvals)
+    singles = np.stack([O.stretch_free_f32(model, g, vp, [n], full=True)[1][:, 0] for n in nvals])
+    stacks = np.stack([O.stretch_free_f32(model, g, vp, [n], full=True)[2][:, 0] for n in nvals])
+    for s in range(g.ns):
+        best, bq = -1.0, -1
+        for q in range(len(nvals)):
+            if singles[q, s] > best:
+                best, bq = singles[q, s], q
+        assert _bits(res[0, s]) == _bits(f32(best)), s
+        if bq >= 0:
+            assert res[2, s] == nvals[bq] and _bits(res[1, s]) == _bits(stacks[bq, s])
+            assert bq != 3    # the repeated shift never wins over its first copy
