@@ -80,6 +80,7 @@ def lib():
         L.hss_status_string.argtypes = [ci]
         L.hss_device_count.argtypes = [ctypes.POINTER(ci)]
         L.hss_verify_index_ops.argtypes = [f32, ctypes.POINTER(u64), vp]
+        L.hss_index_ops_state.argtypes = [f32, ctypes.POINTER(ci)]
         L.hss_build_parameter_array.argtypes = [ci, vp, vp, vp, vp, vp]
         L.hss_grid_semblance.argtypes = [tr, mp, ci, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_grid_semblance_plain.argtypes = [tr, ci, vp, i32, i32, vp, vp, vp, f32, f32, vp,
@@ -108,6 +109,15 @@ def _stream_handle(stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def _on(stream):
+    """Context in which a call's temporaries are allocated (and zeroed) on the stream the
+    call launches on, so the caching allocator orders their reuse after the kernel."""
+    import contextlib
+
+    import torch
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
 
 
 def _ptr(t):
@@ -223,6 +233,14 @@ def verify_index_ops(dt, stream=None):
     return int(n.value)
 
 
+def index_ops_state(dt):
+    """1 when the kernels use the verified short sqrt/division sequences for dt on the
+    current device, 0 when the check failed, -1 when it has not completed yet."""
+    st = ctypes.c_int(0)
+    check(lib().hss_index_ops_state(ctypes.c_float(dt), ctypes.byref(st)))
+    return int(st.value)
+
+
 def build_parameter_array(model, lower, step, count, device="cuda", stream=None):
     import torch
     np_ = MODEL_NPARAMS[model]
@@ -230,7 +248,8 @@ def build_parameter_array(model, lower, step, count, device="cuda", stream=None)
     st = (ctypes.c_float * 3)(*[float(x) for x in step[:np_]] + [0.0] * (3 - np_))
     ct = (ctypes.c_int32 * 3)(*[int(x) for x in count[:np_]] + [1] * (3 - np_))
     P = int(np.prod(count[:np_]))
-    out = torch.empty(P * np_, dtype=torch.float32, device=device)
+    with _on(stream):
+        out = torch.empty(P * np_, dtype=torch.float32, device=device)
     check(lib().hss_build_parameter_array(model, ctypes.cast(lo, ctypes.c_void_p),
                                           ctypes.cast(st, ctypes.c_void_p),
                                           ctypes.cast(ct, ctypes.c_void_p),
@@ -261,14 +280,15 @@ def grid_semblance(line, mids, model, params, P, variant=FAST, full=False,
         raise ValueError("parameter array shorter than P candidates")
     dev = line.samples.device
     np_ = MODEL_NPARAMS[model]
-    if result is None:
-        result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32, device=dev)
     sembl = stack = nu = None
-    if full:
-        sembl = torch.empty((mids.n, line.ns, P), dtype=torch.float32, device=dev)
-        stack = torch.empty((mids.n, line.ns, P), dtype=torch.float32, device=dev)
-    if not_used:
-        nu = torch.zeros(mids.n, dtype=torch.int64, device=dev)
+    with _on(stream):
+        if result is None:
+            result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32, device=dev)
+        if full:
+            sembl = torch.empty((mids.n, line.ns, P), dtype=torch.float32, device=dev)
+            stack = torch.empty((mids.n, line.ns, P), dtype=torch.float32, device=dev)
+        if not_used:
+            nu = torch.zeros(mids.n, dtype=torch.int64, device=dev)
     check(lib().hss_grid_semblance(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
                                    _ptr(params), int(P), int(variant), _ptr(result),
                                    _ptr(sembl), _ptr(stack), _ptr(nu), _stream_handle(stream)))
@@ -286,11 +306,12 @@ def grid_semblance_plain(line, model, params, P, begins, counts, m0s, h0=0.0, ap
     c = np.ascontiguousarray(counts, dtype=np.int32)
     m = np.ascontiguousarray(m0s, dtype=np.float32)
     n = b.size
-    if result is None:
-        result = torch.empty((n, 2 + np_, line.ns), dtype=torch.float32, device="cuda")
-    if scratch is None:
-        scratch = (torch.empty((line.ns, P), dtype=torch.float32, device="cuda"),
-                   torch.empty((line.ns, P), dtype=torch.float32, device="cuda"))
+    with _on(stream):
+        if result is None:
+            result = torch.empty((n, 2 + np_, line.ns), dtype=torch.float32, device="cuda")
+        if scratch is None:
+            scratch = (torch.empty((line.ns, P), dtype=torch.float32, device="cuda"),
+                       torch.empty((line.ns, P), dtype=torch.float32, device="cuda"))
     check(lib().hss_grid_semblance_plain(
         ctypes.byref(line.struct), model, _ptr(params), P, n,
         b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p),
@@ -318,14 +339,15 @@ def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, va
     hi = (ctypes.c_float * 3)(*[float(x) for x in list(upper)[:np_]] + [0.0] * (3 - np_))
     nbytes = de_workspace_bytes(mids.n, line.ns, model, NP)
     g0, g1 = gen_range if gen_range is not None else (0, G)
-    if workspace is None:
-        workspace = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device=dev)
-    if result is None:
-        result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32, device=dev)
     px = pfx = None
-    if population:
-        px = torch.empty((mids.n, line.ns, NP, np_), dtype=torch.float32, device=dev)
-        pfx = torch.empty((mids.n, line.ns, NP, 2), dtype=torch.float32, device=dev)
+    with _on(stream):
+        if workspace is None:
+            workspace = torch.empty(max(nbytes, 4) // 4, dtype=torch.float32, device=dev)
+        if result is None:
+            result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32, device=dev)
+        if population:
+            px = torch.empty((mids.n, line.ns, NP, np_), dtype=torch.float32, device=dev)
+            pfx = torch.empty((mids.n, line.ns, NP, 2), dtype=torch.float32, device=dev)
     check(lib().hss_de_semblance(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
                                  ctypes.cast(lo, ctypes.c_void_p), ctypes.cast(hi, ctypes.c_void_p),
                                  int(NP), int(G), int(g0), int(g1), ctypes.c_uint64(seed),
@@ -347,13 +369,14 @@ def stretch_free(line, mids, model, vparams, nvals, variant=FAST, full=False, no
         raise ValueError("vparams shorter than [n, np, ns]")
     nv = np.ascontiguousarray(nvals, dtype=np.float32)
     nN = nv.size
-    result = torch.empty((mids.n, 3, line.ns), dtype=torch.float32, device=dev)
     sembl = stack = nu = None
-    if full:
-        sembl = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
-        stack = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
-    if not_used:
-        nu = torch.zeros(mids.n, dtype=torch.int64, device=dev)
+    with _on(stream):
+        result = torch.empty((mids.n, 3, line.ns), dtype=torch.float32, device=dev)
+        if full:
+            sembl = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
+            stack = torch.empty((mids.n, line.ns, nN), dtype=torch.float32, device=dev)
+        if not_used:
+            nu = torch.zeros(mids.n, dtype=torch.int64, device=dev)
     check(lib().hss_stretch_free(ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
                                  _ptr(vparams), nv.ctypes.data_as(ctypes.c_void_p), int(nN),
                                  int(variant), _ptr(result), _ptr(sembl), _ptr(stack), _ptr(nu),
@@ -367,8 +390,9 @@ def select_traces(line, model, m0, apm=0.0, h0=0.0, oct_mode=None, pairs=None, v
     import torch
     dev = line.samples.device
     m0 = np.ascontiguousarray(m0, dtype=np.float32)
-    dm0 = torch.from_numpy(m0).to(dev)
-    mask = torch.zeros((m0.size, line.ntr), dtype=torch.uint8, device=dev)
+    with _on(stream):
+        dm0 = torch.from_numpy(m0).to(dev)
+        mask = torch.zeros((m0.size, line.ntr), dtype=torch.uint8, device=dev)
     octc = None
     if model == 2:
         octc = hss_oct_candidates()
@@ -398,7 +422,8 @@ def semblance_gradient(line, mids, model, points, out=None, stream=None):
     [n][(2+np)][ns] (v space).  Returns out [n][(2+np)][ns]: S, stack, gradient rows."""
     import torch
     if out is None:
-        out = torch.empty_like(points)
+        with _on(stream):
+            out = torch.empty_like(points)
     check(lib().hss_semblance_gradient(ctypes.byref(line.struct), ctypes.byref(mids.struct),
                                        int(model), _ptr(points), _ptr(out),
                                        _stream_handle(stream)))
@@ -413,7 +438,8 @@ def polish(line, mids, model, lower, upper, steps, start, result=None, stream=No
     lo = np.ascontiguousarray(lower, dtype=np.float32)
     hi = np.ascontiguousarray(upper, dtype=np.float32)
     if result is None:
-        result = torch.empty_like(start)
+        with _on(stream):
+            result = torch.empty_like(start)
     check(lib().hss_polish(ctypes.byref(line.struct), ctypes.byref(mids.struct), int(model),
                            lo.ctypes.data_as(ctypes.c_void_p), hi.ctypes.data_as(ctypes.c_void_p),
                            int(steps), _ptr(start), _ptr(result), _stream_handle(stream)))

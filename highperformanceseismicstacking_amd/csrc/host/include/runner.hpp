@@ -35,6 +35,13 @@ class SingleHostRunner {
     virtual unsigned int getNumOfDevices() const = 0;
 
   protected:
+    // the steps of main(): read the line and queue its midpoints; run one worker per
+    // device; write the result files; map the exception being handled to an exit status
+    void loadLine();
+    void runWorkers(unsigned int devices);
+    void writeResults(Dumper &dumper);
+    int exitStatusOfFailure() const;
+
     Parser *parser;
     std::mutex resultSetMutex, queueMutex, deviceMutex;
     std::queue<float> midpointQueue;

@@ -17,7 +17,7 @@ ComputeAlgorithm::ComputeAlgorithm(const std::string &name, std::shared_ptr<Trav
 float ComputeAlgorithm::getStatisticalResult(StatisticResult r) const {
     auto it = computedStatisticalResults.find(static_cast<int>(r));
     if (it == computedStatisticalResults.end()) {
-        throw std::invalid_argument("Couldn't find value for statistical result.");
+        throw std::invalid_argument("no value recorded for this statistic");
     }
     return it->second;
 }
@@ -38,8 +38,8 @@ void ComputeAlgorithm::saveStatisticalResults(float totalUsedTracesCount, Second
         static_cast<float>(selectionTime.count());
     st[static_cast<int>(StatisticResult::TOTAL_KERNEL_EXECUTION_TIME)] =
         static_cast<float>(totalTime.count());
-    LOGD("Total execution time for selecting traces is " << selectionTime.count() << "s");
-    LOGI("Total execution time for kernels is " << totalTime.count() << "s");
+    LOGD("trace selection: " << selectionTime.count() << " s in all");
+    LOGI("semblance kernels: " << totalTime.count() << " s in all");
 }
 
 void ComputeAlgorithm::compileKernels(const std::string &path) {
@@ -82,12 +82,12 @@ float LinearSearchAlgorithm::getParameterValueAt(unsigned int iteration, unsigne
 }
 
 void LinearSearchAlgorithm::setDiscretizationDivisorForParameter(unsigned int p, unsigned int d) {
-    if (p >= traveltime->getNumberOfParameters()) throw std::invalid_argument("Parameter index is out of bounds");
+    if (p >= traveltime->getNumberOfParameters()) throw std::invalid_argument("parameter index beyond the model's parameter count");
     discretizationDivisor[p] = d;
 }
 
 void LinearSearchAlgorithm::setDiscretizationGranularityForParameter(unsigned int p, unsigned int g) {
-    if (p >= traveltime->getNumberOfParameters()) throw std::invalid_argument("Parameter index is out of bounds");
+    if (p >= traveltime->getNumberOfParameters()) throw std::invalid_argument("parameter index beyond the model's parameter count");
     discretizationGranularity[p] = g;
 }
 
@@ -119,12 +119,12 @@ void LinearSearchAlgorithm::setUp() {
     copyGatherDataToDevice();
     setupDiscretizationSteps();
     MEASURE_EXEC_TIME(init, initializeParameters());
-    LOGD("initalizationExecutionTime = " << init.count() << "s");
+    LOGD("parameter grid built in " << init.count() << " s");
     isSet = true;
 }
 
 void LinearSearchAlgorithm::computeSemblanceAndParametersForMidpoint(float m0) {
-    LOGI("Computing semblance for m0 = " << m0);
+    LOGI("midpoint " << m0 << ": evaluating");
     const unsigned int ns = Gather::getInstance()->getSamplesPerTrace();
     deviceContext->activate();
     deviceResultArray->reset();
@@ -167,7 +167,7 @@ void DifferentialEvolutionAlgorithm::setUp() {
 }
 
 void DifferentialEvolutionAlgorithm::computeSemblanceAndParametersForMidpoint(float m0) {
-    LOGI("Computing semblance for m0 = " << m0);
+    LOGI("midpoint " << m0 << ": evaluating");
     const unsigned int ns = Gather::getInstance()->getSamplesPerTrace();
     Seconds selection = Seconds::zero(), total = Seconds::zero();
     deviceContext->activate();
@@ -227,7 +227,7 @@ void StretchFreeAlgorithm::readNonStretchedFreeParameterFromFile() {
     std::vector<std::ifstream> files(np);
     for (unsigned int i = 0; i < np; i++) {
         files[i].open(parameterFileArray[i], std::ios::binary);
-        if (!files[i].is_open()) throw std::invalid_argument(parameterFileArray[i] + " couldn't be read.");
+        if (!files[i].is_open()) throw std::invalid_argument("cannot open parameter file " + parameterFileArray[i]);
     }
     for (const auto &kv : gather->getCdps()) {
         std::vector<float> v(static_cast<size_t>(np) * ns, 0.0f);
@@ -244,7 +244,7 @@ void StretchFreeAlgorithm::readNonStretchedFreeParameterFromFile() {
 }
 
 void StretchFreeAlgorithm::computeSemblanceAndParametersForMidpoint(float m0) {
-    LOGI("Computing semblance for m0 = " << m0);
+    LOGI("midpoint " << m0 << ": evaluating");
     const unsigned int ns = Gather::getInstance()->getSamplesPerTrace();
     deviceContext->activate();
     deviceResultArray->reset();

@@ -86,7 +86,8 @@ void HipDataContainer::deallocate() {
 
 void HipDataContainer::copyFrom(const std::vector<float> &source) {
     if (source.size() != elementCount) {
-        throw std::invalid_argument("Allocated memory in GPU is different from source array size.");
+        throw std::invalid_argument("source holds " + std::to_string(source.size()) +
+                                    " elements, the device buffer " + std::to_string(elementCount));
     }
     copyFromWithOffset(source, 0);
 }
@@ -103,7 +104,8 @@ void HipDataContainer::copyFromWithOffset(const std::vector<float> &source, unsi
 
 void HipDataContainer::pasteTo(std::vector<float> &target) {
     if (target.size() != elementCount) {
-        throw std::invalid_argument("Allocated memory in GPU is different from target array size.");
+        throw std::invalid_argument("target holds " + std::to_string(target.size()) +
+                                    " elements, the device buffer " + std::to_string(elementCount));
     }
     if (!elementCount) return;
     HIP_ASSERT(hipMemcpyAsync(target.data(), address, elementCount * sizeof(float),

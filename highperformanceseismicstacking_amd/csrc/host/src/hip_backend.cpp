@@ -353,7 +353,7 @@ void HipLinearSearchAlgorithm::selectTracesToBeUsedForMidpoint(float m0) {
 void HipLinearSearchAlgorithm::computeSemblanceAtGpuForMidpoint(float m0) {
     lastNotUsed = 0;
     if (!filteredTracesCount) {
-        LOGI("No trace has been selected for m0 = " << m0 << ". Skipping.");
+        LOGI("midpoint " << m0 << ": no trace in the aperture, results stay zero");
         return;
     }
     HIP_ASSERT(hipMemsetAsync(dNotUsed.get(), 0, sizeof(unsigned long long), stream()));
@@ -703,7 +703,7 @@ void HipStretchFreeAlgorithm::selectTracesToBeUsedForMidpoint(float m0) {
 void HipStretchFreeAlgorithm::computeSemblanceAtGpuForMidpoint(float m0) {
     lastNotUsed = 0;
     if (!filteredTracesCount) {
-        LOGI("No trace has been selected for m0 = " << m0 << ". Skipping.");
+        LOGI("midpoint " << m0 << ": no trace in the aperture, results stay zero");
         return;
     }
     HIP_ASSERT(hipMemsetAsync(dNotUsed.get(), 0, sizeof(unsigned long long), stream()));

@@ -67,10 +67,10 @@ bool Trace::read(std::ifstream &in, float az) {
     midpoint = Utils::midpoint(info, az);
     halfoffset = Utils::halfoffset(info, az);
     samples.resize(info.ns);
-    LOGD("Reading " << samples.size() * sizeof(float) << " bytes of data");
+    LOGD("trace payload: " << samples.size() * sizeof(float) << " bytes");
     if (!in.read(reinterpret_cast<char *>(samples.data()),
                  static_cast<std::streamsize>(samples.size() * sizeof(float)))) {
-        throw std::runtime_error("end of file reached before reading all samples.");
+        throw std::runtime_error("SU file ends inside the samples of a trace");
     }
     return true;
 }
@@ -111,7 +111,7 @@ void Gather::add(const Trace &trace) {
 void Gather::readGatherFromFile(const std::string &path) {
     std::ifstream in(path, std::ios::binary);
     if (!in.is_open()) {
-        throw std::invalid_argument(path + " couldn't be read.");
+        throw std::invalid_argument("cannot open input file " + path);
     }
     samplesCount = 0;
     bool more = true;
@@ -138,12 +138,12 @@ void Gather::readGatherFromFile(const std::string &path) {
             cdpInfo.offset = 0;
             cdpInfo.sx = cdpInfo.gx = (trace.getInfo().sx + trace.getInfo().gx) >> 1;
             cdpInfo.sy = cdpInfo.gy = (trace.getInfo().sy + trace.getInfo().gy) >> 1;
-            LOGH("CDP not found. Creating a new one with midpoint = " << m);
+            LOGH("new CDP at m = " << m);
             cdps[m] = Cdp(m, cdpInfo);
         }
         if (samplesPerTrace != trace.getSampleCount()) continue;
         if (std::abs(h) > aph) {
-            LOGH("Discarding a trace of CDP (m = " << m << ") (|" << h << "| > " << aph << ")");
+            LOGH("trace at m = " << m << " dropped: |h| = " << std::abs(h) << " beyond aph = " << aph);
             continue;
         }
         if (!trace.isMidpointInRange(lowestAllowedMidpoint, biggestAllowedMidpoint)) continue;
@@ -152,9 +152,8 @@ void Gather::readGatherFromFile(const std::string &path) {
         cdps[m].incrementSampleCountBy(trace.getSampleCount());
     }
     std::sort(traces.begin(), traces.end());
-    LOGI("Number of CDPs = " << cdps.size());
-    LOGI("Number of traces in the gather = " << traces.size());
-    LOGI("Number of samples per trace = " << samplesPerTrace);
+    LOGI("gather: " << cdps.size() << " CDPs, " << traces.size() << " traces of "
+                    << samplesPerTrace << " samples");
     isRead = true;
 }
 

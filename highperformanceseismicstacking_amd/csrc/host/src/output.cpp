@@ -19,7 +19,7 @@ const std::string &statisticName(StatisticResult r) {
 
 const std::vector<float> &MidpointResult::get(float m0) const {
     auto it = values.find(m0);
-    if (it == values.end()) throw std::invalid_argument("Empty result for given midpoint.");
+    if (it == values.end()) throw std::invalid_argument("no result stored for this midpoint");
     return it->second;
 }
 
@@ -30,7 +30,7 @@ void MidpointResult::save(float m0, std::vector<float>::const_iterator begin,
 
 float StatisticalMidpointResult::getStatisticalResultForMidpoint(float m0) const {
     auto it = values.find(m0);
-    if (it == values.end()) throw std::invalid_argument("Empty result for given midpoint.");
+    if (it == values.end()) throw std::invalid_argument("no result stored for this midpoint");
     return it->second;
 }
 
@@ -45,7 +45,7 @@ float StatisticalMidpointResult::getAverageOfAllMidpoints() const {
 
 const StatisticalMidpointResult &ResultSet::get(StatisticResult r) const {
     auto it = statistics.find(static_cast<int>(r));
-    if (it == statistics.end()) throw std::invalid_argument("Empty result for statistical result.");
+    if (it == statistics.end()) throw std::invalid_argument("no midpoint has a value of this statistic");
     return it->second;
 }
 
@@ -72,16 +72,16 @@ Dumper::Dumper(const std::string &path, const std::string &dataFile,
 }
 
 void Dumper::createDir() const {
-    LOGD("Creating output directory at " << outputDirectoryPath);
+    LOGD("output directory: " << outputDirectoryPath);
     if (std::filesystem::exists(outputDirectoryPath)) return;
     if (!std::filesystem::create_directory(outputDirectoryPath)) {
-        throw std::runtime_error("Directory couldn't be created");
+        throw std::runtime_error("cannot create the output directory " + outputDirectoryPath);
     }
 }
 
 static std::ofstream open_or_throw(const std::string &file, const std::string &what) {
     std::ofstream out(file, std::ios::binary);
-    if (!out.is_open()) throw std::runtime_error("Couldn't open " + file + " to write " + what);
+    if (!out.is_open()) throw std::runtime_error("cannot write " + what + " to " + file);
     return out;
 }
 

@@ -280,7 +280,7 @@ ComputeAlgorithm *LinearSearchParser::parseComputeAlgorithm(ComputeAlgorithmBuil
                                                             std::shared_ptr<Traveltime> traveltime) const {
     std::vector<int> granularity;
     if (has("granularity")) granularity = getInts("granularity");
-    LOGD("Received " << granularity.size() << " granularities.");
+    LOGD("grid search over " << granularity.size() << " parameter axes");
     ComputeAlgorithm *a = builder->buildLinearSearchAlgorithm(traveltime, context, getGpuThreadCount(), granularity);
     if (has("kernel-path")) a->compileKernels(getString("kernel-path"));
     return a;
@@ -300,12 +300,11 @@ ComputeAlgorithm *DifferentialEvolutionParser::parseComputeAlgorithm(
     ComputeAlgorithmBuilder *builder, std::shared_ptr<DeviceContext> context,
     std::shared_ptr<Traveltime> traveltime) const {
     if (!has("generations") || !has("population-size")) {
-        throw std::logic_error("Missing parameters for differential evolution.");
+        throw std::logic_error("differential evolution needs --generations and --population-size");
     }
     const unsigned int generations = getUInt("generations");
     const unsigned int individuals = getUInt("population-size");
-    LOGD("Generations = " << generations);
-    LOGD("Individuals per population = " << individuals);
+    LOGD("DE: " << generations << " generations of " << individuals << " individuals");
     ComputeAlgorithm *a = builder->buildDifferentialEvolutionAlgorithm(traveltime, context, getGpuThreadCount(),
                                                                        generations, individuals);
     if (has("kernel-path")) a->compileKernels(getString("kernel-path"));

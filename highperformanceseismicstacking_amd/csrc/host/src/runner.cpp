@@ -4,6 +4,7 @@
 #include <cerrno>
 #include <chrono>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -53,61 +54,78 @@ void SingleHostRunner::workerThread(SingleHostRunner *ref) {
             }
             blocked += std::chrono::steady_clock::now() - t;
         }
-        LOGI("Set up time is " << setUp.count() << " s.");
-        LOGI("Blocked time is " << blocked.count() << " s.");
+        LOGI("worker: set-up " << setUp.count() << " s, waiting on the shared queue and results "
+                               << blocked.count() << " s");
     } catch (...) {
         std::lock_guard<std::mutex> lock(ref->resultSetMutex);
         if (!ref->workerError) ref->workerError = std::current_exception();
     }
 }
 
-int SingleHostRunner::main(int argc, const char *argv[]) {
+namespace {
+
+// Wall-clock time of the named phases of one run, logged together at the end.
+class PhaseTimes {
+  public:
+    template <class F>
+    void run(const char *name, F &&f) {
+        const auto t = std::chrono::steady_clock::now();
+        f();
+        phases.emplace_back(name, std::chrono::steady_clock::now() - t);
+    }
+    void log() const {
+        std::ostringstream s;
+        s << "phases (s):";
+        for (const auto &p : phases) s << ' ' << p.first << ' ' << p.second.count() << ';';
+        LOGI(s.str());
+    }
+
+  private:
+    std::vector<std::pair<const char *, Seconds>> phases;
+};
+
+}  // namespace
+
+void SingleHostRunner::loadLine() {
+    parser->readGather();
+    Gather *gather = Gather::getInstance();
+    // every CDP of the line, in ascending midpoint order, is one unit of work
+    for (const auto &kv : gather->getCdps()) midpointQueue.push(kv.first);
+    resultSet = std::make_unique<ResultSet>(traveltime->getNumberOfResults(), gather->getSamplesPerTrace());
+}
+
+void SingleHostRunner::runWorkers(unsigned int devices) {
+    std::vector<std::thread> pool;
+    pool.reserve(devices);
+    for (unsigned int d = 0; d < devices; d++) pool.emplace_back(workerThread, this);
+    for (auto &t : pool) t.join();
+    if (workerError) std::rethrow_exception(workerError);
+}
+
+void SingleHostRunner::writeResults(Dumper &dumper) {
+    dumper.dumpGatherParameters(parser->getInputFilePath());
+    dumper.dumpTraveltime(traveltime.get());
+    for (unsigned int r = 0; r < traveltime->getNumberOfResults(); r++) {
+        dumper.dumpResult(traveltime->getDescriptionForResult(r), resultSet->getArrayForResult(r));
+    }
+    std::ostringstream means;
+    means << "statistics (mean over midpoints):";
+    for (unsigned int i = 0; i < kStatisticCount; i++) {
+        const StatisticResult r = static_cast<StatisticResult>(i);
+        const StatisticalMidpointResult &values = resultSet->get(r);
+        dumper.dumpStatisticalResult(statisticName(r), values);
+        means << ' ' << statisticName(r) << ' ' << values.getAverageOfAllMidpoints() << ';';
+    }
+    LOGI(means.str());
+}
+
+// The exit status of a run that ended with the exception being handled, as the
+// reference's programs return it (SingleHostRunner.cpp:183-195): a usage error prints the
+// message and the help and exits 0, a runtime error (no device, unreadable file) -ENODEV,
+// any other failure -1.
+int SingleHostRunner::exitStatusOfFailure() const {
     try {
-        const auto start = std::chrono::steady_clock::now();
-        Gather *gather = Gather::getInstance();
-        Seconds initDriver = Seconds::zero(), parseArgs = Seconds::zero(), readGather = Seconds::zero();
-        unsigned int devices = 0;
-        MEASURE_EXEC_TIME(initDriver, devices = getNumOfDevices());
-        MEASURE_EXEC_TIME(parseArgs, parser->parseArguments(argc, argv));
-        traveltime.reset(parser->parseTraveltime());
-        Dumper dumper(parser->getOutputDirectory(), parser->getFilename(), parser->getParserType(),
-                      traveltime->getTraveltimeWord());
-        dumper.createDir();
-        MEASURE_EXEC_TIME(readGather, parser->readGather());
-        for (const auto &kv : gather->getCdps()) midpointQueue.push(kv.first);
-        resultSet = std::make_unique<ResultSet>(traveltime->getNumberOfResults(), gather->getSamplesPerTrace());
-
-        LOGI("Starting worker threads...");
-        const auto workersStart = std::chrono::steady_clock::now();
-        std::vector<std::thread> threads;
-        for (unsigned int d = 0; d < devices; d++) threads.emplace_back(workerThread, this);
-        for (auto &t : threads) t.join();
-        if (workerError) std::rethrow_exception(workerError);
-        const Seconds workers = std::chrono::steady_clock::now() - workersStart;
-
-        const auto dumpStart = std::chrono::steady_clock::now();
-        dumper.dumpGatherParameters(parser->getInputFilePath());
-        dumper.dumpTraveltime(traveltime.get());
-        for (unsigned int i = 0; i < traveltime->getNumberOfResults(); i++) {
-            dumper.dumpResult(traveltime->getDescriptionForResult(i), resultSet->getArrayForResult(i));
-        }
-        for (unsigned int i = 0; i < kStatisticCount; i++) {
-            const StatisticResult r = static_cast<StatisticResult>(i);
-            const StatisticalMidpointResult &sr = resultSet->get(r);
-            dumper.dumpStatisticalResult(statisticName(r), sr);
-            LOGI("Average of " << statisticName(r) << " is " << sr.getAverageOfAllMidpoints());
-        }
-        const Seconds dump = std::chrono::steady_clock::now() - dumpStart;
-        const Seconds total = std::chrono::steady_clock::now() - start;
-        LOGI("Driver init. duration is " << initDriver.count() << " s");
-        LOGI("Read gather duration is " << readGather.count() << " s");
-        LOGI("Parse args. duration is " << parseArgs.count() << " s");
-        LOGI("Worker threads duration is " << workers.count() << " s");
-        LOGI("Dump results duration is " << dump.count() << " s");
-        LOGI("Thread/block is " << parser->getGpuThreadCount());
-        LOGI("Results written to " << dumper.getOutputDirectoryPath());
-        LOGI("It took " << total.count() << " s to compute.");
-        return 0;
+        throw;
     } catch (const std::invalid_argument &e) {
         std::cout << e.what() << std::endl;
         parser->printHelp();
@@ -118,6 +136,36 @@ int SingleHostRunner::main(int argc, const char *argv[]) {
     } catch (const std::exception &e) {
         std::cout << e.what() << std::endl;
         return -1;
+    } catch (...) {
+        std::cout << "unknown failure" << std::endl;
+        return -1;
+    }
+}
+
+int SingleHostRunner::main(int argc, const char *argv[]) {
+    const auto begin = std::chrono::steady_clock::now();
+    try {
+        PhaseTimes phases;
+        unsigned int devices = 0;
+        phases.run("devices", [&] { devices = getNumOfDevices(); });
+        phases.run("arguments", [&] {
+            parser->parseArguments(argc, argv);
+            traveltime.reset(parser->parseTraveltime());
+        });
+        Dumper dumper(parser->getOutputDirectory(), parser->getFilename(), parser->getParserType(),
+                      traveltime->getTraveltimeWord());
+        dumper.createDir();
+        phases.run("read", [&] { loadLine(); });
+        phases.run("search", [&] { runWorkers(devices); });
+        phases.run("write", [&] { writeResults(dumper); });
+        phases.log();
+        const Seconds total = std::chrono::steady_clock::now() - begin;
+        LOGI("Results written to " << dumper.getOutputDirectoryPath());
+        LOGI("run finished in " << total.count() << " s on " << devices << " device(s), "
+                                << parser->getGpuThreadCount() << " threads per block requested");
+        return 0;
+    } catch (...) {
+        return exitStatusOfFailure();
     }
 }
 
@@ -129,7 +177,7 @@ unsigned int HipSingleHostRunner::getNumOfDevices() const {
     int count = 0;
     const int rc = hss_device_count(&count);
     if (rc != HSS_OK || count == 0) {
-        throw std::runtime_error("No HIP devices found on your system. Exiting.");
+        throw std::runtime_error("no HIP device is visible to this process");
     }
     return static_cast<unsigned int>(count);
 }

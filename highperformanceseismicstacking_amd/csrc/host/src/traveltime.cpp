@@ -73,7 +73,7 @@ static void set_bounds(Traveltime *t, const std::vector<float> &lower,
     const unsigned int n = t->getNumberOfParameters();
     if (lower.size() != n || upper.size() != n) {
         std::ostringstream e;
-        e << name << " requires exact " << n << " lower and upper bounds.";
+        e << "the " << name << " traveltime takes " << n << " lower and " << n << " upper bounds";
         delete t;
         throw std::logic_error(e.str());
     }
@@ -94,7 +94,7 @@ Traveltime *TraveltimeBuilder::buildZeroOffsetCommonReflectionSurface(
     const std::vector<float> &lower, const std::vector<float> &upper, float v0, float aIn,
     float bPctg) {
     if (lower.empty() || upper.empty()) {
-        throw std::logic_error("ZOCRS requires at least a single lower/upper bound.");
+        throw std::logic_error("the ZOCRS traveltime needs a velocity lower and upper bound");
     }
     auto *t = new CommonReflectionSurface();
     t->updateLowerBoundForParameter(CommonReflectionSurface::VELOCITY, lower[0]);
@@ -105,8 +105,7 @@ Traveltime *TraveltimeBuilder::buildZeroOffsetCommonReflectionSurface(
     const float vmin = t->getLowerBoundForParameter(CommonReflectionSurface::VELOCITY);
     const float aMax = 2.0f * std::sin(t->getAinInRad()) / v0;
     const float bMax = bPctg * 4 / (vmin * vmin);
-    LOGD("Upper bound for A is " << aMax);
-    LOGD("Upper bound for B is " << bMax);
+    LOGD("ZO-CRS search box: |A| <= " << aMax << ", |B| <= " << bMax);
     t->updateLowerBoundForParameter(CommonReflectionSurface::A, -aMax);
     t->updateUpperBoundForParameter(CommonReflectionSurface::A, aMax);
     t->updateLowerBoundForParameter(CommonReflectionSurface::B, -bMax);

@@ -46,35 +46,56 @@ __global__ __launch_bounds__(256) void verify_index_ops_kernel(float dt, float i
     if (bad) atomicAdd(&verify_mismatches, bad);
 }
 
+namespace {
+std::mutex verify_mu;
+std::map<std::pair<int, unsigned>, bool> verify_cache;  // completed checks only
+}  // namespace
+
 // True when the short sequences may replace sqrtf and t/dt for this (dt, tau) on the
-// current device.  The first call per (device, dt) runs the check and waits for it.
+// current device.  The first call per (device, dt) runs the check and waits for it.  A
+// call on a capturing stream cannot wait, and a HIP call that fails says nothing about
+// the sequences: both answer false (the library operations, the same bits) and cache
+// nothing, so the next call outside capture runs the check.
 bool fast_index_verified(float dt, int tau, hipStream_t stream) {
     // tau >= 1: no window accepts t/dt < 1; dt >= 2^-40: a traveltime below 2^-48 (a
     // radicand below 2^-96, where sqrt_rn_big may differ from sqrtf) has t/dt < 1
     if (tau < 1 || !(dt >= 0x1.0p-40f)) return false;
-    static std::mutex mu;
-    static std::map<std::pair<int, unsigned>, bool> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     unsigned bits;
     __builtin_memcpy(&bits, &dt, 4);
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = cache.find({dev, bits});
-    if (it != cache.end()) return it->second;
+    std::lock_guard<std::mutex> lock(verify_mu);
+    auto it = verify_cache.find({dev, bits});
+    if (it != verify_cache.end()) return it->second;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return false;
     const float inv = 1.0f / dt;
     unsigned long long zero = 0, bad = 1;
-    bool ok = hipMemcpyToSymbolAsync(HIP_SYMBOL(verify_mismatches), &zero, sizeof(zero), 0,
-                                     hipMemcpyHostToDevice, stream) == hipSuccess;
-    if (ok) {
-        hipLaunchKernelGGL(verify_index_ops_kernel, dim3(8192), dim3(256), 0, stream, dt, inv, 1);
-        ok = hipGetLastError() == hipSuccess &&
-             hipMemcpyFromSymbolAsync(&bad, HIP_SYMBOL(verify_mismatches), sizeof(bad), 0,
-                                      hipMemcpyDeviceToHost, stream) == hipSuccess &&
-             hipStreamSynchronize(stream) == hipSuccess;
-    }
-    ok = ok && bad == 0;
-    cache[{dev, bits}] = ok;
+    if (hipMemcpyToSymbolAsync(HIP_SYMBOL(verify_mismatches), &zero, sizeof(zero), 0,
+                               hipMemcpyHostToDevice, stream) != hipSuccess)
+        return false;
+    hipLaunchKernelGGL(verify_index_ops_kernel, dim3(8192), dim3(256), 0, stream, dt, inv, 1);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyFromSymbolAsync(&bad, HIP_SYMBOL(verify_mismatches), sizeof(bad), 0,
+                                 hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return false;
+    const bool ok = bad == 0;
+    verify_cache[{dev, bits}] = ok;
     return ok;
+}
+
+// 1: the check for dt passed on the current device; 0: it found a mismatch; -1: it has
+// not completed there yet.
+int index_ops_state(float dt) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    unsigned bits;
+    __builtin_memcpy(&bits, &dt, 4);
+    std::lock_guard<std::mutex> lock(verify_mu);
+    auto it = verify_cache.find({dev, bits});
+    return it == verify_cache.end() ? -1 : (it->second ? 1 : 0);
 }
 
 }  // namespace hss
@@ -96,4 +117,12 @@ extern "C" int hss_verify_index_ops(float dt, unsigned long long *mismatches, vo
                                  hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return (int)e;
     return hip_status(hipStreamSynchronize(st));
+}
+
+// Whether the semblance entry points use the short sequences for dt on the current device.
+extern "C" int hss_index_ops_state(float dt, int *state) {
+    if (!state) return HSS_ERR_NULL_POINTER;
+    if (!(dt > 0.0f)) return HSS_ERR_INVALID_SIZE;
+    *state = hss::index_ops_state(dt);
+    return HSS_OK;
 }

@@ -18,7 +18,8 @@
  *     where a declaration says "allocates" (none in this version);
  *   - the first semblance call on a device for a given dt runs a one-time check of the
  *     short square-root and division sequences the kernels use (hss_verify_index_ops)
- *     and waits for it; later calls are fully asynchronous;
+ *     and waits for it (outside stream capture: hss_index_ops_state); later calls are
+ *     fully asynchronous;
  *   - floating point in/out is float32; trace samples may be stored as float32,
  *     float16 or bfloat16 (hss_traces.storage) with a per-trace float32 scale.
  *
@@ -120,6 +121,14 @@ int hss_device_count(int *count);
  * tau >= 1, and the library operations otherwise; results are the same bits either way.
  * Synchronous on `stream`. */
 int hss_verify_index_ops(float dt, unsigned long long *mismatches, void *stream);
+
+/* Diagnostics (no counterpart in the reference): the state of that check for dt on the
+ * current device -- *state = 1 when it passed (the kernels use the short sequences),
+ * 0 when it found a mismatch (they use the library operations), -1 when it has not run
+ * to completion there yet.  A semblance call on a stream under capture does not run the
+ * check (it cannot wait): it uses the library operations and leaves the state at -1, so
+ * the first call outside capture runs it. */
+int hss_index_ops_state(float dt, int *state);
 
 /* Grid candidates in storage form: CMP {c}, ZO-CRS {c,A,B}, OCT {c,slope}, c = 4/v^2,
  * v-major (ZO-CRS: B fastest; OCT: slope fastest); value d of axis k is

@@ -343,4 +343,4 @@ def test_missing_input_prints_help_and_returns_zero(tmp_path):
                           "--traveltime", "cmp", "--lower-bounds", "1500", "--upper-bounds", "4500",
                           "--granularity", "8"], capture_output=True, text=True, timeout=60)
     assert res.returncode == 0
-    assert "couldn't be read" in res.stdout and "Allowed options" in res.stdout
+    assert "cannot open input file" in res.stdout and "Allowed options" in res.stdout

@@ -79,3 +79,40 @@ def test_de_graph_replay_equals_eager():
         assert torch.equal(out, eager)
         capi.de_semblance(dl, mp, ZOCRS, lo, hi, NP, G, seed + 1, variant=FAST)
         torch.cuda.synchronize()
+
+
+def test_first_call_under_capture_then_eager_uses_verified_sequences():
+    """A first semblance call for a new dt made under stream capture cannot run the
+    one-time sequence check (it would wait on the capturing stream): it takes the library
+    operations (same bits) and caches nothing, so the next eager call runs the check and
+    the kernels use the verified short sequences from then on."""
+    import torch
+    dt_us = 3000                      # a sample interval no other test uses
+    line = synth.make_line(3, 24, 2000.0, 401, dt_us, seed=12, offset_min=50.0)
+    tau = synth.tau_index(0.012, dt_us)
+    G = 64
+    step = float(np.float32(np.float32(4500.0 - 1500.0) / np.float32(G)))
+    b, c, m0 = line.cmp_ranges()
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    mp = capi.DeviceMidpoints(b, c, m0)
+    params = capi.build_parameter_array(CMP, [1500.0], [step], [G])
+    torch.cuda.synchronize()
+    assert capi.index_ops_state(line.dt) == -1
+    s = torch.cuda.Stream()
+    out = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        capi.grid_semblance(dl, mp, CMP, params, G, variant=FAST, result=out, stream=s)
+    assert capi.index_ops_state(line.dt) == -1
+    graph.replay()
+    torch.cuda.synchronize()
+    eager, _, _, _ = capi.grid_semblance(dl, mp, CMP, params, G, variant=FAST)
+    torch.cuda.synchronize()
+    assert capi.index_ops_state(line.dt) == 1
+    assert torch.equal(out, eager)
+    for _ in range(2):
+        out.fill_(-7.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)

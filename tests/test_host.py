@@ -77,14 +77,14 @@ def test_su_reader_errors(host, tmp_path):
     args = [np.zeros(1, t) for t in (np.int32, np.float32, np.int32, np.int32, np.int32, np.int32)]
     rc = host.hss_host_read_gather(str(tmp_path / "missing.su").encode(), 1.0, 0.0, 0.02, 90.0,
                                    None, None, *[_p(a) for a in args])
-    assert rc == -1 and b"couldn't be read" in host.hss_host_last_error()
+    assert rc == -1 and b"cannot open input file" in host.hss_host_last_error()
     # a record cut inside its samples is a runtime error (Trace.cpp:58-60)
     raw = open(os.path.join(GOLD, "gather_basic.su"), "rb").read()
     cut = tmp_path / "cut.su"
     cut.write_bytes(raw[: 240 + 4 * 32 + 240 + 40])
     rc = host.hss_host_read_gather(str(cut).encode(), 1e4, 0.0, 0.02, 90.0, None, None,
                                    *[_p(a) for a in args])
-    assert rc == -3 and b"end of file" in host.hss_host_last_error()
+    assert rc == -3 and b"ends inside the samples" in host.hss_host_last_error()
 
 
 def test_traveltime_bounds_match_reference_builder(host):
