@@ -177,6 +177,13 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
         if constexpr (MODEL == HSS_OCT) o_l = oct_trace<FASTIDX>(g_l.h, g_l.m, e.h0);
         const int nchunk = min(kWave, e.cnt - j0);
         for (int jj = 0; jj < nchunk; ++jj) {
+            if constexpr (VARIANT == HSS_KAHAN) {
+                // a compensation chunk ends every kKahanChunk traces (wave-uniform)
+                if (((j0 + jj) & (kKahanChunk - 1)) == 0 && j0 + jj > 0) {
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc[c].flush(e.w);
+                }
+            }
             const int ti = __builtin_amdgcn_readlane(ti_l, jj);
             TraceGeom tg{0.0f, 0.0f, 0.0f};
             OctTrace ot{};

@@ -10,9 +10,12 @@
 //             of the reference text bit for bit.
 //   HSS_FAST  u = fma(y1 - y0, frac, y0); A_j += u; den = fma(u, u, den); the linear sum
 //             is recovered at the end as sum_j A_j (same value up to rounding).
-//   HSS_KAHAN u as HSS_FAST; every A_j and den carry a Kahan compensation term
-//             (den's also takes the rounding error of u*u, through the FMA that forms
-//             u*u - c), and the final sums over j are compensated too.
+//   HSS_KAHAN two-level compensation: the values of up to kKahanChunk consecutive
+//             traces are summed as HSS_FAST sums them (A_j += u, den = fma(u, u, den)),
+//             then each chunk's sums are Kahan-added into running totals (flush()), and
+//             the final sums over j are compensated too.  A chunk is a sum of a few
+//             terms, so its rounding stays at a few ulps whatever the number of traces,
+//             and the compensation costs a few operations per chunk, not per trace.
 #pragma once
 
 #include "device_common.hpp"
@@ -94,24 +97,48 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
     s = t;
 }
 
+// Traces per compensation chunk of HSS_KAHAN (a power of two).
+constexpr int kKahanChunk = 8;
+
 template <int W, int VARIANT>
 struct Window {
     static constexpr int N = W > 0 ? W : kMaxWindow;
     static constexpr bool KAHAN = VARIANT == HSS_KAHAN;
-    float A[N];
-    float Ac[KAHAN ? N : 1];
-    float lin, den, denc;
+    static constexpr int NK = KAHAN ? N : 1;
+    float A[N];            // window sums (HSS_KAHAN: of the current chunk)
+    float T[NK], Tc[NK];   // HSS_KAHAN: running totals of A_j and their compensation
+    float lin, den, tden, denc;
     unsigned used;
 
     __device__ __forceinline__ void reset() {
 #pragma unroll
         for (int j = 0; j < N; ++j) A[j] = 0.0f;
 #pragma unroll
-        for (int j = 0; j < (KAHAN ? N : 1); ++j) Ac[j] = 0.0f;
+        for (int j = 0; j < NK; ++j) {
+            T[j] = 0.0f;
+            Tc[j] = 0.0f;
+        }
         lin = 0.0f;
         den = 0.0f;
+        tden = 0.0f;
         denc = 0.0f;
         used = 0;
+    }
+
+    // HSS_KAHAN: adds the chunk sums to the compensated totals and starts a new chunk
+    // (a no-op for the other variants).  Callers flush every kKahanChunk traces, at
+    // wave-uniform points of the trace loop.
+    __device__ __forceinline__ void flush(int w) {
+        if constexpr (KAHAN) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (W == 0 && j >= w) continue;
+                kahan_add(T[j], Tc[j], A[j]);
+                A[j] = 0.0f;
+            }
+            kahan_add(tden, denc, den);
+            den = 0.0f;
+        }
     }
 
     // One interpolated sample u = y + frac*d at window position j, d = y_next - y (the
@@ -122,18 +149,11 @@ struct Window {
             A[j] += u;
             lin += u;
             den += u * u;
-        } else if constexpr (VARIANT == HSS_FAST) {
+        } else {
+            // HSS_FAST, and HSS_KAHAN inside a chunk
             float u = __builtin_fmaf(d, f, y);
             A[j] += u;
             den = __builtin_fmaf(u, u, den);
-        } else {
-            float u = __builtin_fmaf(d, f, y);
-            kahan_add(A[j], Ac[j], u);
-            // den + u*u with compensation: y' = u*u - c in one rounding
-            const float yy = __builtin_fmaf(u, u, -denc);
-            const float t = den + yy;
-            denc = (t - den) - yy;
-            den = t;
         }
     }
 
@@ -179,7 +199,7 @@ struct Window {
     __device__ __forceinline__ void add16(const uint32_t *roww, unsigned rpar, unsigned k,
                                           float f, float sc) {
         static_assert(W > 0, "add16 needs a compiled window size");
-        if constexpr (VARIANT == HSS_FAST && STORAGE == HSS_F16) {
+        if constexpr (VARIANT != HSS_EXACT && STORAGE == HSS_F16) {
             // as below, with y1 - y0 formed as fma(y1, one, -y0) = RN(y1 - y0): the same
             // bits, but an FMA reads both halves straight from the words (v_fma_mix_f32),
             // where a subtraction needs both converted first.  `one` is 1.0 hidden from the
@@ -203,7 +223,7 @@ struct Window {
         }
         float y[W + 1];
         read16<STORAGE, W>(roww, rpar, k, y);
-        if constexpr (VARIANT == HSS_FAST) {
+        if constexpr (VARIANT != HSS_EXACT) {
             // the trace's scale factored out of the window: u = sc * (y0 + frac*(y1 - y0))
             // on the stored values, A_j += sc*u and den += sc^2 * sum_j u^2
             float dn = 0.0f;
@@ -237,8 +257,9 @@ struct Window {
     }
 
     // REDUCE_SEMBLANCE_STACK: S = sum_j A_j^2 / (used * den), stack = lin / (used * w);
-    // both stay 0 when no trace was used.
-    __device__ __forceinline__ void reduce(int w, float *S, float *stack) const {
+    // both stay 0 when no trace was used.  (HSS_KAHAN: the last chunk is flushed first.)
+    __device__ __forceinline__ void reduce(int w, float *S, float *stack) {
+        flush(w);
         float sem = 0.0f, stk = 0.0f;
         if (used > 0) {
             float num = 0.0f, numc = 0.0f;
@@ -253,8 +274,9 @@ struct Window {
                     num = __builtin_fmaf(A[j], A[j], num);
                     l += A[j];
                 } else {
-                    // A_j carries its compensation: use the corrected value
-                    const float a = A[j] - Ac[j];
+                    // the total carries its compensation: use the corrected value; the
+                    // square's rounding goes into the compensation through the FMA
+                    const float a = T[j] - Tc[j];
                     const float y = __builtin_fmaf(a, a, -numc);
                     const float t = num + y;
                     numc = (t - num) - y;
@@ -266,7 +288,7 @@ struct Window {
             if constexpr (KAHAN) {
                 num = num - numc;
                 l = l - lc;
-                d = den - denc;
+                d = tden - denc;
             }
             sem = num / ((float)used * d);
             stk = l / (float)(used * (unsigned)w);
