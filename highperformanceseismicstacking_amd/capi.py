@@ -88,6 +88,8 @@ def lib():
         L.hss_de_workspace_size.argtypes = [i32, i32, ci, i32, ctypes.POINTER(u64)]
         L.hss_de_semblance.argtypes = [tr, mp, ci, vp, vp, i32, i32, i32, i32, u64, u32, ci, vp,
                                        u64, vp, vp, vp, vp, vp]
+        L.hss_de_semblance_noalloc.argtypes = [tr, mp, ci, vp, vp, i32, i32, i32, i32, u64, u32,
+                                               ci, vp, vp, vp, vp, vp]
         L.hss_stretch_free.argtypes = [tr, mp, ci, vp, vp, i32, ci, vp, vp, vp, vp, vp]
         L.hss_semblance_gradient.argtypes = [tr, mp, ci, vp, vp, vp]
         L.hss_grid_set_config.argtypes = [ci, ci, ci, ci, i32]
@@ -355,6 +357,32 @@ def de_semblance(line, mids, model, lower, upper, NP, G, seed, stream_base=0, va
                                  ctypes.c_uint64(workspace.numel() * 4), _ptr(result), _ptr(px),
                                  _ptr(pfx), _ptr(not_used), _stream_handle(stream)))
     return result, px, pfx
+
+
+DE_SCRATCH_BYTES = 64 << 20   # HSS_DE_SCRATCH_BYTES
+
+
+def de_semblance_noalloc(line, mids, model, lower, upper, NP, G, seed, stream_base=0,
+                         variant=FAST, result=None, stream=None, gen_range=None, not_used=None):
+    """hss_de_semblance_noalloc: DE with the populations in the library's static device
+    scratch (no workspace argument).  `result` must be given by a caller that cannot
+    allocate; here it is allocated when None.  Returns result [n,(2+np),ns]."""
+    import torch
+    check_ranges(line, mids)
+    np_ = MODEL_NPARAMS[model]
+    lo = (ctypes.c_float * 3)(*[float(x) for x in list(lower)[:np_]] + [0.0] * (3 - np_))
+    hi = (ctypes.c_float * 3)(*[float(x) for x in list(upper)[:np_]] + [0.0] * (3 - np_))
+    g0, g1 = gen_range if gen_range is not None else (0, G)
+    if result is None:
+        with _on(stream):
+            result = torch.empty((mids.n, 2 + np_, line.ns), dtype=torch.float32,
+                                 device=line.samples.device)
+    check(lib().hss_de_semblance_noalloc(
+        ctypes.byref(line.struct), ctypes.byref(mids.struct), model,
+        ctypes.cast(lo, ctypes.c_void_p), ctypes.cast(hi, ctypes.c_void_p), int(NP), int(G),
+        int(g0), int(g1), ctypes.c_uint64(seed), ctypes.c_uint32(stream_base), int(variant),
+        _ptr(result), None, None, _ptr(not_used), _stream_handle(stream)))
+    return result
 
 
 def stretch_free(line, mids, model, vparams, nvals, variant=FAST, full=False, not_used=False,

@@ -113,6 +113,8 @@ const char *hss_status_string(int status) {
         case HSS_ERR_INVALID_STORAGE: return "invalid sample storage type";
         case HSS_ERR_INVALID_VARIANT: return "invalid arithmetic variant";
         case HSS_ERR_NO_DEVICE: return "no HIP device found";
+        case HSS_ERR_SCRATCH_LIMIT:
+            return "batch needs more DE workspace than the static scratch holds (HSS_DE_SCRATCH_BYTES)";
         default:
             if (status > 0) return hipGetErrorString((hipError_t)status);
             return "unknown status";

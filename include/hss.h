@@ -60,7 +60,8 @@ enum hss_status {
     HSS_ERR_NULL_POINTER = -4,
     HSS_ERR_INVALID_STORAGE = -5,  /* unknown storage, or samples not 4-byte aligned */
     HSS_ERR_INVALID_VARIANT = -6,
-    HSS_ERR_NO_DEVICE = -7
+    HSS_ERR_NO_DEVICE = -7,
+    HSS_ERR_SCRATCH_LIMIT = -8     /* the call needs more than the static scratch holds */
 };
 
 /* All traces of a line (or of one midpoint), resident on the device, in the order the
@@ -212,6 +213,26 @@ int hss_de_semblance(const hss_traces *traces, const hss_midpoints *mids, int mo
                      int variant, void *d_workspace, uint64_t workspace_bytes,
                      float *d_result, float *d_pop_x, float *d_pop_fx,
                      unsigned long long *d_not_used, void *stream);
+
+/* Size of the static device scratch array of hss_de_semblance_noalloc (64 MiB; one copy
+ * per device that calls it, held until the process ends). */
+#define HSS_DE_SCRATCH_BYTES (64ull << 20)
+
+/* hss_de_semblance without a caller workspace (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The populations live in a file-scope device
+ * array of HSS_DE_SCRATCH_BYTES; the call allocates and frees nothing.  A batch whose
+ * hss_de_workspace_size exceeds HSS_DE_SCRATCH_BYTES is refused with
+ * HSS_ERR_SCRATCH_LIMIT before anything runs (split the midpoints into smaller batches).
+ * Arguments and results are those of hss_de_semblance, bit for bit.  Every call on a
+ * device shares that device's array and it keeps the last call's populations (a run split
+ * over generation ranges resumes from them when no other call intervenes); calls that
+ * overlap on one device, on any streams, race. */
+int hss_de_semblance_noalloc(const hss_traces *traces, const hss_midpoints *mids, int model,
+                             const float *lower, const float *upper, int32_t NP, int32_t G,
+                             int32_t gen_begin, int32_t gen_end, uint64_t seed,
+                             uint32_t stream_base, int variant, float *d_result, float *d_pop_x,
+                             float *d_pop_fx, unsigned long long *d_not_used, void *stream);
 
 /* Stretch-free re-stack for a batch of midpoints.  d_vparams [n][np][ns] holds, per
  * sample, the parameters picked by an earlier search (velocity in v space, then A,B or

@@ -33,7 +33,7 @@ def test_status_strings_and_version_without_gpu():
     from highperformanceseismicstacking_amd import capi
     lib = capi.lib()
     assert lib.hss_abi_version() >= 1
-    for code in [0, -1, -2, -3, -4, -5, -6, -7]:
+    for code in [0, -1, -2, -3, -4, -5, -6, -7, -8]:
         assert lib.hss_status_string(code)
 
 
