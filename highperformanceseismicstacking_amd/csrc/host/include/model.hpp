@@ -12,8 +12,12 @@
 //    midpoint is FLT_MIN, so midpoints <= 0 are dropped; traces end sorted by (m, h).
 //  * The reference's read loop makes one extra read at end of file, which inserts a CDP
 //    at m = 0 unless one exists.  That CDP is kept here too (it has no traces and is
-//    processed and written like the others); its header, uninitialised in the
-//    reference, is zero here except ns and dt.
+//    processed and written like the others).  Its header is never read in the
+//    reference (a default-constructed Trace leaves it uninitialised), and the reference
+//    built with g++ leaves the previous record's header bytes in it; the reader here
+//    gives it exactly those bytes (the last record's header, then offset = 0 and the
+//    source/receiver coordinates averaged as for every CDP), so the written files match
+//    the reference's byte for byte.
 #pragma once
 
 #include <cstdint>
@@ -69,9 +73,11 @@ class Trace {
     const std::vector<float> &getSamples() const { return samples; }
     bool isMidpointInRange(float lower, float upper) const;
 
-    // Reads one record.  Returns false (leaving the trace empty) when the stream holds
-    // no further header; throws std::runtime_error when the samples are truncated.
-    bool read(std::ifstream &in, float azimuthInRad);
+    // Reads one record.  Returns false (leaving the trace without samples, at m = h = 0)
+    // when the stream holds no further header; its header then holds `previous` with
+    // whatever bytes a partial header read overwrote.  Throws std::runtime_error when the
+    // samples are truncated.
+    bool read(std::ifstream &in, float azimuthInRad, const trace_info_t &previous);
 
     bool operator<(const Trace &other) const;
 

@@ -233,7 +233,7 @@ void StretchFreeAlgorithm::readNonStretchedFreeParameterFromFile() {
         std::vector<float> v(static_cast<size_t>(np) * ns, 0.0f);
         for (unsigned int i = 0; i < np; i++) {
             Trace t;
-            if (!t.read(files[i], gather->getAzimuthInRad()) || t.getSampleCount() != ns) {
+            if (!t.read(files[i], gather->getAzimuthInRad(), trace_info_t{}) || t.getSampleCount() != ns) {
                 throw std::runtime_error(parameterFileArray[i] + ": no trace of " + std::to_string(ns) +
                                          " samples for the CDP at m = " + std::to_string(kv.first));
             }

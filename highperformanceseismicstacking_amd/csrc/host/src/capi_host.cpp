@@ -12,6 +12,7 @@
 
 #include "logger.hpp"
 #include "model.hpp"
+#include "output.hpp"
 #include "parser.hpp"
 #include "traveltime.hpp"
 
@@ -108,6 +109,55 @@ int hss_host_xcd_balanced_order(const int32_t *work, int32_t n, int32_t *order) 
         const std::vector<int32_t> w(work, work + n);
         const std::vector<int32_t> o = xcd_balanced_order(w);
         std::copy(o.begin(), o.end(), order);
+    });
+}
+
+int hss_host_dump_results(const char *out_path, const char *input_path, const char *data_name,
+                          const char *method, int model, const float *lower, const float *upper,
+                          int32_t nbounds, float v0, float ain, float bpctg, float h0,
+                          const float *values, const float *statistics, char *dir,
+                          int32_t dir_capacity) {
+    return guarded([&] {
+        if (!out_path || !input_path || !data_name || !method || !values || !statistics)
+            throw std::invalid_argument("null argument");
+        if (nbounds < 0 || (nbounds > 0 && (!lower || !upper)))
+            throw std::invalid_argument("bounds missing");
+        const Gather *g = Gather::getInstance();
+        std::vector<float> lo(lower, lower + nbounds), hi(upper, upper + nbounds);
+        TraveltimeBuilder *b = TraveltimeBuilder::getInstance();
+        std::unique_ptr<Traveltime> tt;
+        if (model == CMP) tt.reset(b->buildCommonMidPoint(lo, hi));
+        else if (model == ZOCRS) tt.reset(b->buildZeroOffsetCommonReflectionSurface(lo, hi, v0, ain, bpctg));
+        else if (model == OCT) tt.reset(b->buildOffsetContinuationTrajectory(lo, hi, h0));
+        else throw std::invalid_argument("Invalid traveltime model");
+        const unsigned nres = tt->getNumberOfResults(), ns = g->getSamplesPerTrace();
+        const unsigned nstat = static_cast<unsigned>(StatisticResult::CNT);
+        ResultSet results(nres, ns);
+        size_t k = 0;
+        for (const auto &kv : g->getCdps()) {
+            const float *v = values + k * nres * ns;
+            results.setAllResultsForMidpoint(kv.first, std::vector<float>(v, v + nres * ns));
+            for (unsigned s = 0; s < nstat; s++)
+                results.setStatisticalResultForMidpoint(kv.first, static_cast<StatisticResult>(s),
+                                                        statistics[k * nstat + s]);
+            k++;
+        }
+        Dumper dumper(out_path, data_name, method, tt->getTraveltimeWord());
+        dumper.createDir();
+        dumper.dumpGatherParameters(input_path);
+        dumper.dumpTraveltime(tt.get());
+        for (unsigned i = 0; i < nres; i++)
+            dumper.dumpResult(tt->getDescriptionForResult(i), results.getArrayForResult(i));
+        for (unsigned s = 0; s < nstat; s++) {
+            const StatisticResult r = static_cast<StatisticResult>(s);
+            dumper.dumpStatisticalResult(statisticName(r), results.get(r));
+        }
+        const std::string &d = dumper.getOutputDirectoryPath();
+        if (dir && dir_capacity > 0) {
+            const size_t n = std::min(d.size(), static_cast<size_t>(dir_capacity - 1));
+            std::memcpy(dir, d.data(), n);
+            dir[n] = 0;
+        }
     });
 }
 

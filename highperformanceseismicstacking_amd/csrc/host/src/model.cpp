@@ -60,7 +60,8 @@ bool Trace::isMidpointInRange(float lower, float upper) const {
     return midpoint >= lower && midpoint <= upper;
 }
 
-bool Trace::read(std::ifstream &in, float az) {
+bool Trace::read(std::ifstream &in, float az, const trace_info_t &previous) {
+    info = previous;
     if (!in.read(reinterpret_cast<char *>(&info), sizeof(trace_info_t))) {
         return false;
     }
@@ -115,11 +116,14 @@ void Gather::readGatherFromFile(const std::string &path) {
     }
     samplesCount = 0;
     bool more = true;
+    trace_info_t previous{};
     while (more) {
         Trace trace;
-        more = trace.read(in, azimuthInRad);
-        // The record after the last trace is an empty trace at m = 0: it still creates a
-        // CDP entry, as the reference's loop does (see model.hpp).
+        more = trace.read(in, azimuthInRad, previous);
+        previous = trace.getInfo();
+        // The record after the last trace is an empty trace at m = 0 carrying the last
+        // record's header: it still creates a CDP entry, as the reference's loop does
+        // (see model.hpp).
         const float m = trace.getMidpoint();
         const float h = trace.getHalfoffset();
         if (!samplesPerTrace) {
@@ -131,10 +135,6 @@ void Gather::readGatherFromFile(const std::string &path) {
         }
         if (cdps.find(m) == cdps.end()) {
             trace_info_t cdpInfo = trace.getInfo();
-            if (!more) {
-                cdpInfo.ns = static_cast<uint16_t>(samplesPerTrace);
-                cdpInfo.dt = dt;
-            }
             cdpInfo.offset = 0;
             cdpInfo.sx = cdpInfo.gx = (trace.getInfo().sx + trace.getInfo().gx) >> 1;
             cdpInfo.sy = cdpInfo.gy = (trace.getInfo().sy + trace.getInfo().gy) >> 1;

@@ -35,6 +35,21 @@ int hss_host_traveltime_bounds(int model, const float *lower, const float *upper
                                float v0, float ain, float bpctg, float *out_lower,
                                float *out_upper);
 
+/* Writes a result set with the output layer of the programs (Dumper, the reference's
+ * common/src/output/Dumper.cpp:13-150 re-implemented) for the gather last read by
+ * hss_host_read_gather: the directory <out_path>/<data_name>_<method>_<traveltime word>_
+ * <YYYYMMDD_HHMM>/ with one SU file per result (CDP header + ns float32 per CDP, in
+ * midpoint order), one CSV per statistic, gather.txt (input_path as given) and
+ * traveltime.txt -- the files SingleHostRunner::main writes after a run.  The traveltime is
+ * built as TraveltimeBuilder builds it (model 0 CMP, 1 ZO-CRS with v0/ain/bpctg, 2 OCT
+ * with h0).  values [ncdp][results][ns] and statistics [ncdp][5] follow the CDP map order.
+ * The directory path goes to dir (NUL-terminated, truncated to dir_capacity). */
+int hss_host_dump_results(const char *out_path, const char *input_path, const char *data_name,
+                          const char *method, int model, const float *lower, const float *upper,
+                          int32_t nbounds, float v0, float ain, float bpctg, float h0,
+                          const float *values, const float *statistics, char *dir,
+                          int32_t dir_capacity);
+
 /* Parses a command line of program 0 (single_host_linear_search), 1 (single_host_de) or
  * 2 (single_host_stretch_free) and builds its traveltime, without reading data or
  * opening a device. */

@@ -9,8 +9,14 @@ reference's code produced for them.
                     OCT computeDisplacedMidpoint / computeTime, for CMP, ZO-CRS and OCT;
   oct_functions.npz OCT displaced midpoint / traveltime with their error codes;
   gather_*.su/.npz  SU files and what the reference's Gather::readGatherFromFile keeps;
-  traveltime.npz    ZO-CRS / OCT parameter bounds from TraveltimeBuilder.
+  traveltime.npz    ZO-CRS / OCT parameter bounds from TraveltimeBuilder;
+  dump_<gather>_<model>/  the output files the reference's own Dumper (with ResultSet and the
+                    result classes) writes for a gather_*.su file, a traveltime model and a
+                    seeded result set (values.f32 [cdp][result][ns], stats.f32 [cdp][5],
+                    args.json), in the order SingleHostRunner::main writes them.
 """
+import json
+import shutil
 import ctypes
 import os
 import subprocess
@@ -209,6 +215,56 @@ def make_traveltime():
                         zocrs=np.array(rows, np.float64))
 
 
+DUMP_CASES = [
+    # (gather, read args (aph, apm, tau, azimuth), model, lower, upper, v0, ain, bpctg, h0, method)
+    ("basic", (500.0, 0.0, 0.02, 90.0), "cmp", [1500.0], [4500.0], 0.0, 0.0, 0.0, 0.0, "greedy"),
+    ("basic", (500.0, 0.0, 0.02, 90.0), "zocrs", [1600.0], [4800.0], 2000.0, 60.0, 0.5, 0.0, "de"),
+    ("basic", (500.0, 0.0, 0.02, 90.0), "oct", [1500.0, -5e-4], [4500.0, 5e-4], 0.0, 0.0, 0.0,
+     100.0, "greedy"),
+    ("filters", (400.0, 0.0, 0.022, 90.0), "cmp", [1400.0], [5000.0], 0.0, 0.0, 0.0, 0.0, "de"),
+]
+NRESULTS = {"cmp": 3, "zocrs": 5, "oct": 4}
+
+
+def make_dump_fixtures():
+    """Runs the reference's Dumper on each case with the SU file named relative to
+    tests/golden (so gather.txt reads 'Input file = gather_<name>.su')."""
+    for ci, (gname, (aph, apm, tau, az), model, lo, hi, v0, ain, bp, h0, method) in enumerate(DUMP_CASES):
+        su = f"gather_{gname}.su"
+        cdps = np.load(os.path.join(HERE, f"gather_{gname}.npz"))["cdps"]
+        ns = int(np.load(os.path.join(HERE, f"gather_{gname}.npz"))["ns"])
+        rng = np.random.default_rng(700 + ci)
+        values = rng.normal(0.0, 1.0, (len(cdps), NRESULTS[model], ns)).astype(np.float32)
+        # statistics of the magnitudes a run produces (efficiency, interpolations/s, traces,
+        # seconds), so that the CSV text takes fixed and scientific forms
+        stats = (rng.uniform(0.0, 1.0, (len(cdps), 5)) *
+                 np.array([1.0, 3e9, 120.0, 2e-3, 0.25])).astype(np.float32)
+        stats[:, 2] = np.round(stats[:, 2])
+        out = os.path.join(HERE, f"dump_{gname}_{model}")
+        shutil.rmtree(out, ignore_errors=True)
+        os.makedirs(out)
+        values.tofile(os.path.join(out, "values.f32"))
+        stats.tofile(os.path.join(out, "stats.f32"))
+        tmp = os.path.join("/tmp", f"golden_dump_{ci}")
+        shutil.rmtree(tmp, ignore_errors=True)
+        os.makedirs(tmp)
+        args = ["dump", su, str(aph), str(apm), str(tau), str(az), model, str(len(lo))]
+        args += [repr(x) for x in lo] + [repr(x) for x in hi]
+        args += [str(v0), str(ain), str(bp), str(h0), tmp, f"gather_{gname}", method,
+                 os.path.join(out, "values.f32"), os.path.join(out, "stats.f32")]
+        res = subprocess.run([os.path.join(REF, "ref_host")] + args, cwd=HERE, check=True,
+                             capture_output=True, text=True)
+        written = res.stdout.strip()
+        for f in sorted(os.listdir(written)):
+            shutil.copy(os.path.join(written, f), os.path.join(out, f))
+        json.dump({"gather": su, "aph": aph, "apm": apm, "tau": tau, "azimuth": az,
+                   "model": model, "lower": lo, "upper": hi, "v0": v0, "ain": ain, "bpctg": bp,
+                   "h0": h0, "method": method, "data_name": f"gather_{gname}",
+                   "directory": os.path.basename(written.rstrip("/"))[:-len("YYYYMMDD_HHMM")],
+                   "files": sorted(os.listdir(written))},
+                  open(os.path.join(out, "args.json"), "w"), indent=1)
+
+
 def main():
     if not os.path.exists(os.path.join(REF, "libref_kernel.so")):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle", "ref")], check=True)
@@ -224,6 +280,7 @@ def main():
     make_gather_fixture("filters", line, aph=400.0, apm=0.0, tau=0.022, az=90.0,
                         lowest="1010", highest="1040", shuffle_seed=4, bad_ns_rows=(2, 7))
     make_traveltime()
+    make_dump_fixtures()
     print("golden fixtures written to", HERE)
 
 

@@ -344,3 +344,55 @@ def test_missing_input_prints_help_and_returns_zero(tmp_path):
                           "--granularity", "8"], capture_output=True, text=True, timeout=60)
     assert res.returncode == 0
     assert "cannot open input file" in res.stdout and "Allowed options" in res.stdout
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("case", ["dump_basic_cmp", "dump_basic_zocrs", "dump_basic_oct",
+                                  "dump_filters_cmp"])
+def test_program_side_files_and_headers_equal_the_reference_dumper(case, tmp_path):
+    """The programs' output files against the reference's own Dumper (fixtures written by
+    the reference-built checker, tests/golden/dump_*): gather.txt and traveltime.txt byte
+    for byte, every SU result's CDP headers (240 bytes each, the end-of-file CDP's
+    included) byte for byte, the CSV files' midpoint column as text, the file set and the
+    directory name up to its timestamp.  (The values themselves are the GPU's and are
+    checked against the oracle by the other tests.)"""
+    import json
+    import shutil
+    a = json.load(open(os.path.join(GOLD, case, "args.json")))
+    shutil.copy(os.path.join(GOLD, a["gather"]), tmp_path / a["gather"])
+    out = tmp_path / "out"
+    out.mkdir()
+    args = ["--aph", a["aph"], "--apm", a["apm"], "--azimuth", a["azimuth"], "--tau", a["tau"],
+            "--input", a["gather"], "--output", str(out), "--traveltime", a["model"],
+            "--lower-bounds", *a["lower"], "--upper-bounds", *a["upper"]]
+    if a["model"] == "zocrs":
+        args += ["--v0", a["v0"], "--ain", a["ain"], "--bpctg", a["bpctg"]]
+    if a["model"] == "oct":
+        args += ["--h0", a["h0"]]
+    if a["method"] == "greedy":
+        prog = "single_host_linear_search"
+        args += ["--granularity"] + [8] * len(a["lower"])
+    else:
+        prog = "single_host_de"
+        args += ["--generations", 4, "--population-size", 8, "--seed", 5]
+    outdir, _ = run(prog, args, tmp_path)
+    name = os.path.basename(outdir.rstrip("/"))
+    assert name.startswith(a["directory"]), (name, a["directory"])
+    assert sorted(os.listdir(outdir)) == a["files"]
+    for f in a["files"]:
+        got = open(os.path.join(outdir, f), "rb").read()
+        want = open(os.path.join(GOLD, case, f), "rb").read()
+        if f.endswith(".txt"):
+            assert got == want, f
+        elif f.endswith(".su"):
+            assert len(got) == len(want), f
+            ns = int(np.load(os.path.join(GOLD, a["gather"].replace(".su", ".npz")))["ns"])
+            n = len(want) // (240 + 4 * ns)
+            for k in range(n):
+                r0 = k * (240 + 4 * ns)
+                assert got[r0:r0 + 240] == want[r0:r0 + 240], (f, k)
+        else:
+            col = lambda b: [ln.split(b",")[0] for ln in b.split(b"\n") if ln]
+            assert col(got) == col(want), f

@@ -1,9 +1,12 @@
 """The C++17 host layer on the CPU (libhss_host.so through include/hss_host.h): the SU
 reader against what the reference's own Gather::readGatherFromFile kept for the same
 files (fixtures written by the reference-built checker), the traveltime bounds against
-the reference's TraveltimeBuilder, and the command-line rules."""
+the reference's TraveltimeBuilder, the output files byte for byte against the reference's
+own Dumper, and the command-line rules."""
 import ctypes
+import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -26,6 +29,9 @@ def host():
     L.hss_host_traveltime_bounds.argtypes = [ctypes.c_int, vp, vp, ctypes.c_int32, f, f, f, vp, vp]
     L.hss_host_parse_command_line.argtypes = [ctypes.c_int, ctypes.c_int, vp]
     L.hss_host_xcd_balanced_order.argtypes = [vp, ctypes.c_int32, vp]
+    L.hss_host_dump_results.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_int, vp, vp,
+                                        ctypes.c_int32, f, f, f, f, vp, vp, ctypes.c_char_p,
+                                        ctypes.c_int32]
     return L
 
 
@@ -195,3 +201,40 @@ def test_xcd_balanced_order(host, n):
     sums = [int(work[b].sum()) for b in bins]
     assert max(sums) - min(sums) <= 2 * int(work.max(initial=0))
     assert host.hss_host_xcd_balanced_order(_p(work), -1, _p(order)) == -1
+
+
+DUMP_CASES = sorted(d for d in os.listdir(GOLD) if d.startswith("dump_"))
+
+
+@pytest.mark.parametrize("case", DUMP_CASES)
+def test_output_files_equal_the_reference_dumper_byte_for_byte(host, case, tmp_path,
+                                                               monkeypatch):
+    """The output layer writes the same bytes as the reference's Dumper (fixtures written
+    by the reference-built checker from common/src/output/Dumper.cpp, ResultSet.cpp and
+    the result classes) for the same gather, traveltime and result set: every SU file
+    (CDP headers, the end-of-file CDP's included, and samples), every CSV (default stream
+    formatting of m0 and the statistic), gather.txt and traveltime.txt; only the
+    directory's timestamp differs."""
+    fix = os.path.join(GOLD, case)
+    a = json.load(open(os.path.join(fix, "args.json")))
+    monkeypatch.chdir(GOLD)                      # gather.txt names the input as given
+    read_gather(host, a["gather"], a["aph"], a["apm"], a["tau"], a["azimuth"])
+    values = np.fromfile(os.path.join(fix, "values.f32"), np.float32)
+    stats = np.fromfile(os.path.join(fix, "stats.f32"), np.float32)
+    lo = np.array(a["lower"], np.float32)
+    hi = np.array(a["upper"], np.float32)
+    model = {"cmp": 0, "zocrs": 1, "oct": 2}[a["model"]]
+    buf = ctypes.create_string_buffer(4096)
+    rc = host.hss_host_dump_results(str(tmp_path).encode(), a["gather"].encode(),
+                                    a["data_name"].encode(), a["method"].encode(), model,
+                                    _p(lo), _p(hi), len(lo), a["v0"], a["ain"], a["bpctg"],
+                                    a["h0"], _p(values), _p(stats), buf, 4096)
+    assert rc == 0, host.hss_host_last_error()
+    out = buf.value.decode()
+    name = os.path.basename(out.rstrip("/"))
+    assert name.startswith(a["directory"]) and re.fullmatch(r"\d{8}_\d{4}", name[len(a["directory"]):])
+    assert sorted(os.listdir(out)) == a["files"]
+    for f in a["files"]:
+        got = open(os.path.join(out, f), "rb").read()
+        want = open(os.path.join(fix, f), "rb").read()
+        assert got == want, f
