@@ -176,14 +176,19 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
         OctTrace o_l{};
         if constexpr (MODEL == HSS_OCT) o_l = oct_trace<FASTIDX>(g_l.h, g_l.m, e.h0);
         const int nchunk = min(kWave, e.cnt - j0);
-        for (int jj = 0; jj < nchunk; ++jj) {
-            if constexpr (VARIANT == HSS_KAHAN) {
-                // a compensation chunk ends every kKahanChunk traces (wave-uniform)
-                if (((j0 + jj) & (kKahanChunk - 1)) == 0 && j0 + jj > 0) {
+        // HSS_KAHAN: the trace loop runs in blocks of kKahanChunk traces (j0 is a multiple
+        // of kWave, itself a multiple of the chunk), each block's sums flushed into the
+        // compensated totals before the next (wave-uniform)
+        constexpr int JB = VARIANT == HSS_KAHAN ? kKahanChunk : kWave;
+        for (int jb = 0; jb < nchunk; jb += JB) {
+        if constexpr (VARIANT == HSS_KAHAN) {
+            if (j0 + jb > 0) {
 #pragma unroll
-                    for (int c = 0; c < CPL; ++c) acc[c].flush(e.w);
-                }
+                for (int c = 0; c < CPL; ++c) acc[c].flush(e.w);
             }
+        }
+        const int jend = min(jb + JB, nchunk);
+        for (int jj = jb; jj < jend; ++jj) {
             const int ti = __builtin_amdgcn_readlane(ti_l, jj);
             TraceGeom tg{0.0f, 0.0f, 0.0f};
             OctTrace ot{};
@@ -354,7 +359,8 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                         acc[c].add(rd, trace0 + (size_t)(kk[c] - e.tau), f[c], sc, scaled, e.w);
                 }
             }
-        }
+        }  // jj
+        }  // jb
     }
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].reduce(e.w, &S[c], &stack[c]);

@@ -23,6 +23,12 @@ namespace hss {
 constexpr int kGridWaves = 4;
 constexpr int kGridThreads = kGridWaves * kWave;
 
+// 16-bit storage reads windows through the wave's LDS segments; per-lane reads from the
+// trace row (Window::add16, what DE and stretch-free use) measured 45% slower for the grid
+// on the C5 line (FP16, 2048 velocities), FAST and Kahan alike.
+template <int STORAGE>
+constexpr bool grid_segments() { return STORAGE != HSS_F32; }
+
 struct GridArgs {
     EvalCtx e;  // per-midpoint fields are filled in by the kernel
     const int *mbeg;
@@ -85,8 +91,18 @@ __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, 
 // CPL: candidates per lane (the parameter tile): wave v takes the candidate groups
 // [q0, q0 + CPL) with q0 = CPL*v, CPL*(v + 4), ...; within a lane candidates are taken in
 // increasing order, so the running best is still the first strict maximum.
+// Register budget of the compensated variant, as waves per SIMD the compiler must fit:
+// its running totals and their compensation double the window state, which at one
+// candidate per lane would leave 5 waves; 7 fit without spilling and measured 12% faster
+// on the C5 line (FP16).  (Two candidates per lane do not fit well at any budget; the
+// autotuner picks one.)
+template <int VARIANT, int CPL>
+constexpr int grid_min_waves() { return VARIANT == HSS_KAHAN && CPL == 1 ? 7 : 1; }
+
 template <int MODEL, int W, int VARIANT, int STORAGE, int CPL>
-__global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a) {
+__global__ __launch_bounds__(kGridThreads)
+__attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void grid_semblance_kernel(
+    GridArgs a) {
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -117,8 +133,8 @@ __global__ __launch_bounds__(kGridThreads) void grid_semblance_kernel(GridArgs a
         }
         float S[CPL], stk[CPL];
         unsigned rej_q = 0;
-        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL>(e, t0, t0sq, k, on, 0.0f, active,
-                                                           seg_lds[wave], S, stk, &rej_q);
+        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, grid_segments<STORAGE>()>(
+            e, t0, t0sq, k, on, 0.0f, active, seg_lds[wave], S, stk, &rej_q);
         if (active) {
             rejected += rej_q;
 #pragma unroll

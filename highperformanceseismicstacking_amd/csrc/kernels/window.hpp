@@ -97,8 +97,12 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
     s = t;
 }
 
-// Traces per compensation chunk of HSS_KAHAN (a power of two).
-constexpr int kKahanChunk = 8;
+// Traces per compensation chunk of HSS_KAHAN (a power of two that divides the 64 traces of
+// a geometry chunk).  16 measured 4% faster than 8 on the C5 line (FP16 grid); a chunk's
+// plain sum then rounds at most about 8 ulps, against the ~36 of a plain 72-trace sum.
+constexpr int kKahanChunk = 16;
+static_assert((kKahanChunk & (kKahanChunk - 1)) == 0 && kKahanChunk <= 64,
+              "the chunk must be a power of two dividing 64");
 
 template <int W, int VARIANT>
 struct Window {
