@@ -177,6 +177,8 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     a.params = d_params;
     a.P = P;
     a.ntiles = tiles_of(traces->ns);
+    a.tt = 1;
+    a.wg_per_mid = a.ntiles;
     a.result = d_result;
     a.sembl = d_sembl;
     a.stack = d_stack;
@@ -184,22 +186,24 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     if ((unsigned long long)mids->n * (unsigned long long)a.ntiles > 0x7fffffffULL)
         return HSS_ERR_INVALID_SIZE;
     hipStream_t st = (hipStream_t)stream;
-    auto run = [&](const GridArgs &ga, int cpl) {
+    auto run = [&](const GridArgs &ga, GridShape sh, int slots) {
+        GridArgs la = ga;
+        la.tt = sh.tt;
+        la.wg_per_mid = (a.ntiles + sh.tt - 1) / sh.tt;
         switch (model) {
-            case HSS_CMP: return launch_grid_cmp(ga, mids->n, variant, traces->storage, cpl, st);
-            case HSS_ZOCRS: return launch_grid_zocrs(ga, mids->n, variant, traces->storage, cpl, st);
-            default: return launch_grid_oct(ga, mids->n, variant, traces->storage, cpl, st);
+            case HSS_CMP: return launch_grid_cmp(la, slots, variant, traces->storage, sh.cpl, st);
+            case HSS_ZOCRS: return launch_grid_zocrs(la, slots, variant, traces->storage, sh.cpl, st);
+            default: return launch_grid_oct(la, slots, variant, traces->storage, sh.cpl, st);
         }
     };
     // timing runs must not add to the accumulating rejection counts
     GridArgs probe = a;
     probe.not_used = nullptr;
-    int cpl = 1;
-    rc = tuned_grid_cpl(model, traces->w, variant, traces->storage,
-                        (unsigned)mids->n * (unsigned)a.ntiles, st,
-                        [&](int c) { return run(probe, c); }, &cpl);
+    GridShape shape{1, 1};
+    rc = tuned_grid_shape(model, traces->w, variant, traces->storage, mids->n, a.ntiles, st,
+                          [&](GridShape sh, int slots) { return run(probe, sh, slots); }, &shape);
     if (rc != HSS_OK) return rc;
-    return run(a, cpl);
+    return run(a, shape, mids->n);
 }
 
 int hss_de_workspace_size(int32_t n_mid, int32_t ns, int model, int32_t NP, uint64_t *bytes) {

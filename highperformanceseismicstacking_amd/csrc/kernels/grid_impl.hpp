@@ -38,7 +38,9 @@ struct GridArgs {
     int nmid;
     const float *params;
     int P;
-    int ntiles;
+    int ntiles;      // 64-sample t0 tiles of a midpoint
+    int tt;          // t0 tiles per workgroup (1, 2 or 4; the waves split into tt x 4/tt)
+    int wg_per_mid;  // workgroups per midpoint, ceil(ntiles / tt)
     float *result;
     float *sembl;
     float *stack;
@@ -61,11 +63,11 @@ __device__ __forceinline__ Candidate load_candidate(const float *__restrict__ pa
     return k;
 }
 
-// Merges the per-wave running bests of a workgroup (larger value first, then lower
-// index); returns the winner in wave 0's lanes.
+// Merges the running bests of the waves that share a t0 tile (waves tloc, tloc + tt, ...;
+// larger value first, then lower index); returns the winner in the lanes of wave tloc.
 template <int NWAVES>
 __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, int wave,
-                                            int lane, float *B, float *BS, int *BI) {
+                                            int lane, int tt, float *B, float *BS, int *BI) {
     __shared__ float sh_best[NWAVES][kWave];
     __shared__ float sh_stack[NWAVES][kWave];
     __shared__ int sh_idx[NWAVES][kWave];
@@ -76,8 +78,8 @@ __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, 
     *B = -1.0f;
     *BS = 0.0f;
     *BI = -1;
-#pragma unroll
-    for (int v = 0; v < NWAVES; ++v) {
+    const int tloc = wave % tt;
+    for (int v = tloc; v < NWAVES; v += tt) {
         const int iv = sh_idx[v][lane];
         const float sv = sh_best[v][lane];
         if (iv >= 0 && (sv > *B || (sv == *B && iv < *BI))) {
@@ -88,9 +90,9 @@ __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, 
     }
 }
 
-// CPL: candidates per lane (the parameter tile): wave v takes the candidate groups
-// [q0, q0 + CPL) with q0 = CPL*v, CPL*(v + 4), ...; within a lane candidates are taken in
-// increasing order, so the running best is still the first strict maximum.
+// CPL: candidates per lane (the parameter tile): wave group v of the 4/tt groups takes the
+// candidates [q0, q0 + CPL) with q0 = CPL*v, CPL*(v + 4/tt), ...; within a lane candidates
+// are taken in increasing order, so the running best is still the first strict maximum.
 // Register budget of the compensated variant, as waves per SIMD the compiler must fit:
 // its running totals and their compensation double the window state, which at one
 // candidate per lane would leave 5 waves; 7 fit without spilling and measured 12% faster
@@ -106,14 +108,19 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
+    // the workgroup covers tt t0 tiles; its waves split into tt tiles x 4/tt candidate
+    // groups (wave = group * tt + tile)
+    const int tt = a.tt;
+    const int ngroups = kGridWaves / tt;
+    const int grp = wave / tt;
     const int blk = logical_block(blockIdx.x, gridDim.x);
-    const int slot = blk / a.ntiles;
-    const int tile = blk - slot * a.ntiles;
+    const int slot = blk / a.wg_per_mid;
+    const int tile = (blk - slot * a.wg_per_mid) * tt + wave % tt;
     const int g = launch_midpoint(a.morder, a.nmid, slot);
     if (g < 0) return;
     const int s = tile * kWave + lane;
     const int ns = a.e.ns;
-    const bool active = s < ns;
+    const bool active = s < ns;  // (a tile past the last one has no active lane)
     const float t0 = (float)s * a.e.dt;
     const float t0sq = t0 * t0;
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
@@ -123,7 +130,7 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
     int bidx = -1;
     unsigned long long rejected = 0;
 
-    for (int q0 = wave * CPL; q0 < a.P; q0 += kGridWaves * CPL) {
+    for (int q0 = grp * CPL; q0 < a.P; q0 += ngroups * CPL) {
         Candidate k[CPL];
         bool on[CPL];
 #pragma unroll
@@ -159,8 +166,8 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
 
     float B, BS;
     int BI;
-    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, &B, &BS, &BI);
-    if (wave != 0 || !active) return;
+    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, tt, &B, &BS, &BI);
+    if (grp != 0 || !active) return;
     float *res = a.result + (size_t)g * (2 + NP) * ns;
     res[s] = B;
     res[ns + s] = BI >= 0 ? BS : 0.0f;
@@ -175,12 +182,13 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
     }
 }
 
-// Host launcher for one model: picks the window/variant/storage instantiation.
+// Host launcher for one model: picks the window/variant/storage instantiation.  n_slots
+// launch slots (midpoints taken up in launch order) of a.wg_per_mid workgroups each.
 template <int MODEL>
-int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+int launch_grid(const GridArgs &a, int n_slots, int variant, int storage, int cpl,
                 hipStream_t stream) {
     int st = HSS_OK;
-    const unsigned blocks = (unsigned)n_mid * (unsigned)a.ntiles;
+    const unsigned blocks = (unsigned)n_slots * (unsigned)a.wg_per_mid;
     if (blocks == 0) return HSS_OK;
     int rc = dispatch_storage(storage, [&](auto S) {
         st = dispatch_variant(variant, [&](auto V) {
@@ -201,11 +209,11 @@ int launch_grid(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
     return hip_status(hipGetLastError());
 }
 
-int launch_grid_cmp(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+int launch_grid_cmp(const GridArgs &a, int n_slots, int variant, int storage, int cpl,
                     hipStream_t stream);
-int launch_grid_zocrs(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+int launch_grid_zocrs(const GridArgs &a, int n_slots, int variant, int storage, int cpl,
                     hipStream_t stream);
-int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+int launch_grid_oct(const GridArgs &a, int n_slots, int variant, int storage, int cpl,
                     hipStream_t stream);
 
 }  // namespace hss

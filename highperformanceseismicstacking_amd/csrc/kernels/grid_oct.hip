@@ -2,8 +2,8 @@
 #include "grid_impl.hpp"
 
 namespace hss {
-int launch_grid_oct(const GridArgs &a, int n_mid, int variant, int storage, int cpl,
+int launch_grid_oct(const GridArgs &a, int n_slots, int variant, int storage, int cpl,
                     hipStream_t stream) {
-    return launch_grid<HSS_OCT>(a, n_mid, variant, storage, cpl, stream);
+    return launch_grid<HSS_OCT>(a, n_slots, variant, storage, cpl, stream);
 }
 }  // namespace hss

@@ -45,10 +45,18 @@ inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; 
 // first time).
 bool fast_index_verified(float dt, int tau, hipStream_t stream);
 
-// Grid-search parameter tile (candidates per lane) for this key, tuned on the first large
-// launch by timing launch(cpl) for each choice (tune.hip).
-int tuned_grid_cpl(int model, int w, int variant, int storage, unsigned blocks, hipStream_t stream,
-                   const std::function<int(int)> &launch, int *cpl_out);
+// Grid-search launch shape: candidates per lane (cpl, 1 or 2) and 64-sample t0 tiles per
+// workgroup (tt, 1, 2 or 4).  Results do not depend on it.
+struct GridShape {
+    int cpl;
+    int tt;
+};
+
+// The shape for this key, tuned on the first large launch by timing launch(shape, slots)
+// for each shape on the first `slots` launch slots of the batch (tune.hip).
+int tuned_grid_shape(int model, int w, int variant, int storage, int n_slots, int ntiles,
+                     hipStream_t stream, const std::function<int(GridShape, int)> &launch,
+                     GridShape *shape_out);
 
 // Calls f(std::integral_constant<int, W>) for the compiled window sizes; any other w
 // (odd, <= 32), or fast = false, goes to the runtime-w instantiation W = 0.

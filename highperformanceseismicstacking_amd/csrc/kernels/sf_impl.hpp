@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
     }
     float B, BS;
     int BI;
-    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, &B, &BS, &BI);
+    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, 1, &B, &BS, &BI);
     if (wave != 0 || !active) return;
     float *res = a.result + (size_t)g * 3 * ns;
     res[s] = B;

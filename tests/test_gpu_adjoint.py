@@ -93,8 +93,6 @@ def test_gradient_matches_fp64_oracle(model, storage):
                 bad += 1
                 fails.append((g, int(samples[i]), pts[g][2:, samples[i]].tolist(), float(S32[i]),
                               S64[i], G32[i].tolist(), G64[i].tolist()))
-    np.save(f"gpurun_out/adjoint_fails_{model}_{storage}.npy", np.array(fails, dtype=object),
-            allow_pickle=True) if fails else None
     assert bad == 0, (bad, total, fails[:4])
 
 

@@ -183,9 +183,11 @@ def test_plain_per_gather_kernel_equals_batched_exact(model):
 
 @pytest.mark.parametrize("storage", [F32, BF16])
 @pytest.mark.parametrize("variant", [EXACT, FAST, KAHAN])
-def test_parameter_tile_does_not_change_results(storage, variant):
-    """Two candidates per lane (the autotuner's other shape) give the one-candidate
-    results bit for bit, including an odd candidate count (a half-empty last tile)."""
+def test_launch_shape_does_not_change_results(storage, variant):
+    """Every launch shape of the autotuner (1 or 2 candidates per lane, 1, 2 or 4 t0 tiles
+    per workgroup) gives the one-candidate, one-tile results bit for bit, including an odd
+    candidate count (a half-empty last parameter tile) and 13 t0 tiles (workgroups whose
+    last waves have no tile)."""
     import torch
     line = synth.make_line(5, (30, 60), 3000.0, 801, 2000, seed=21, offset_min=50.0)
     tau = synth.tau_index(0.011, line.dt_us)
@@ -194,14 +196,19 @@ def test_parameter_tile_does_not_change_results(storage, variant):
     b, c, m0 = line.cmp_ranges()
     w = 2 * tau + 1
     out = {}
+    assert (line.ns + 63) // 64 == 13
     try:
         for cpl in (1, 2):
-            capi.grid_set_config(CMP, w, variant, storage, cpl)
-            out[cpl] = run_gpu(line, b, c, m0, CMP, params, tau, variant, storage=storage)
+            for tt in (1, 2, 4):
+                capi.grid_set_shape(CMP, w, variant, storage, cpl, tt)
+                assert capi.grid_tuned_shape(CMP, w, variant, storage) == (cpl, tt)
+                out[(cpl, tt)] = run_gpu(line, b, c, m0, CMP, params, tau, variant,
+                                         storage=storage)
     finally:
-        capi.grid_set_config(CMP, w, variant, storage, 0)
-    for i in range(3):
-        assert np.array_equal(out[1][i].view(np.uint32), out[2][i].view(np.uint32)), i
+        capi.grid_set_shape(CMP, w, variant, storage, 0, 0)
+    for key, r in out.items():
+        for i in range(3):
+            assert np.array_equal(out[(1, 1)][i].view(np.uint32), r[i].view(np.uint32)), (key, i)
 
 
 def test_line_batch_equals_single_midpoints():
