@@ -18,7 +18,7 @@
 // coalesced row; one workgroup barrier separates generations.
 #pragma once
 
-#include "eval.hpp"
+#include "eval_compact.hpp"
 #include "grid_impl.hpp"
 #include "launch_util.hpp"
 
@@ -60,6 +60,10 @@ constexpr bool kDeSegments = false;
 // (m, h) order, where nothing is shared, the launch is about 11% slower than a kernel that
 // does not keep them (measured on the C4 line).
 constexpr bool kDeOctKeep = true;
+// OCT: the aperture test of a run of traces first, then each lane's kept traces only
+// (eval_compact.hpp); same bits as evaluate_n.  Not for HSS_KAHAN.
+template <int MODEL, int VARIANT>
+constexpr bool de_compact() { return MODEL == HSS_OCT && kDeOctKeep && VARIANT != HSS_KAHAN; }
 constexpr float kDeF = 0.85f;   // F_FAC, common/include/gpu/interface.h:5
 constexpr float kDeCR = 0.5f;   // CR, common/include/gpu/interface.h:6
 
@@ -164,13 +168,23 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     };
 
     __shared__ float seg_lds[kDeWaves][kSegFloats];
+    __shared__ OctTraceRec trace_recs[kDeWaves][kWave];
     unsigned long long rejected = 0;
+    constexpr int CPL = de_cpl<MODEL, W, VARIANT>();
+    auto eval = [&](const Candidate (&kc)[CPL], const bool (&on)[CPL], float (&S)[CPL],
+                    float (&stk)[CPL], unsigned *rej) {
+        if constexpr (de_compact<MODEL, VARIANT>())
+            evaluate_oct_compact<W, VARIANT, STORAGE, false, CPL>(
+                e, t0, kc, on, 0.0f, active, trace_recs[wave], S, stk, rej);
+        else
+            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep>(
+                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, rej);
+    };
     // Generation g's population lives in buffer g & 1 of the workspace, so a run split
     // over several launches ([gen_begin, gen_end) each) gives the same result as one.
     // generation 0: the start population (startPopulations) and its fitness
     // A wave evaluates CPL individuals i0, i0+V, ... at a time (one trace loop for all);
     // a slot past NP is evaluated on individual i0's data and discarded.
-    constexpr int CPL = de_cpl<MODEL, W, VARIANT>();
     for (int i0 = wave; a.gen_begin == 0 && i0 < NP; i0 += kDeWaves * CPL) {
         float x[CPL][D];
         Candidate kc[CPL];
@@ -187,8 +201,7 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
         }
         float S[CPL], stk[CPL];
         unsigned rej = 0;
-        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep>(
-            e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
+        eval(kc, on, S, stk, &rej);
         if (active) rejected += rej;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
@@ -239,8 +252,7 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
             }
             float S[CPL], stk[CPL];
             unsigned rej = 0;
-            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep>(
-                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
+            eval(kc, on, S, stk, &rej);
             if (active) rejected += rej;
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
 #pragma unroll
