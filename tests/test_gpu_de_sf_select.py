@@ -101,7 +101,7 @@ def test_de_batch_equals_single_midpoints():
         assert np.array_equal(_bits(res[g].cpu().numpy()), _bits(r1[0].cpu().numpy()))
 
 
-@pytest.mark.parametrize("model", [CMP, ZOCRS])
+@pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
 def test_de_fast_statistics_and_consistency(model):
     import torch
     line, b, c, m0, lo, hi, h0, apm, tau = de_case(model)
@@ -112,7 +112,8 @@ def test_de_fast_statistics_and_consistency(model):
     torch.cuda.synchronize()
     res = res.cpu().numpy()
     for g in range(len(b)):
-        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), apm=apm)
+        og = oracle_gather(line.samples, line, int(b[g]), int(c[g]), tau, float(m0[g]), h0=h0,
+                           apm=apm)
         r = O.de_f32(model, og, lo, hi, NP, G, seed, stream_id=g)
         close = np.abs(res[g, 0] - r[0]) <= 1e-3
         assert close.mean() >= 0.97, close.mean()
