@@ -82,3 +82,9 @@ class Dumper {
   private:
     std::string outputDirectoryPath;
 };
+
+// The files a finished run leaves (SingleHostRunner.cpp:150-170, SpitzCommitter.cpp:
+// 65-80): gather.txt with the input path, traveltime.txt, one SU file per result and one
+// CSV per statistic, in that order, into the dumper's (already created) directory.
+void dumpResultSet(const Dumper &dumper, const std::string &inputPath, const Traveltime &model,
+                   const ResultSet &results);

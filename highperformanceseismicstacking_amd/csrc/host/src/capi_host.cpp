@@ -2,6 +2,7 @@
 #include "hss_host.h"
 #include "hip_backend.hpp"
 #include "semblance_op.hpp"
+#include "task.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -37,6 +38,14 @@ static int guarded(F &&f) {
         last_error = e.what();
         return -4;
     }
+}
+
+// A path into a caller's buffer, NUL-terminated and truncated to its capacity.
+static void copy_path(const std::string &d, char *dir, int32_t dir_capacity) {
+    if (!dir || dir_capacity <= 0) return;
+    const size_t n = std::min(d.size(), static_cast<size_t>(dir_capacity - 1));
+    std::memcpy(dir, d.data(), n);
+    dir[n] = 0;
 }
 
 extern "C" {
@@ -144,20 +153,8 @@ int hss_host_dump_results(const char *out_path, const char *input_path, const ch
         }
         Dumper dumper(out_path, data_name, method, tt->getTraveltimeWord());
         dumper.createDir();
-        dumper.dumpGatherParameters(input_path);
-        dumper.dumpTraveltime(tt.get());
-        for (unsigned i = 0; i < nres; i++)
-            dumper.dumpResult(tt->getDescriptionForResult(i), results.getArrayForResult(i));
-        for (unsigned s = 0; s < nstat; s++) {
-            const StatisticResult r = static_cast<StatisticResult>(s);
-            dumper.dumpStatisticalResult(statisticName(r), results.get(r));
-        }
-        const std::string &d = dumper.getOutputDirectoryPath();
-        if (dir && dir_capacity > 0) {
-            const size_t n = std::min(d.size(), static_cast<size_t>(dir_capacity - 1));
-            std::memcpy(dir, d.data(), n);
-            dir[n] = 0;
-        }
+        dumpResultSet(dumper, input_path, *tt, results);
+        copy_path(dumper.getOutputDirectoryPath(), dir, dir_capacity);
     });
 }
 
@@ -174,6 +171,92 @@ int hss_host_parse_command_line(int program, int argc, const char **argv) {
     });
 }
 
+
+// ---- per-midpoint tasks ----------------------------------------------------------------
+struct hss_host_job {
+    std::unique_ptr<TaskJob> job;
+};
+struct hss_host_worker {
+    std::unique_ptr<TaskWorker> worker;
+};
+
+int hss_host_job_open(int program, int argc, const char **argv, hss_host_job **job) {
+    return guarded([&] {
+        if (!job) throw std::invalid_argument("null job handle");
+        *job = nullptr;
+        auto j = std::make_unique<hss_host_job>();
+        j->job = std::make_unique<TaskJob>(program, argc, argv);
+        *job = j.release();
+    });
+}
+
+int hss_host_job_record_sizes(const hss_host_job *job, int32_t *task_bytes, int32_t *result_bytes,
+                              int32_t *tasks) {
+    return guarded([&] {
+        if (!job || !task_bytes || !result_bytes || !tasks) throw std::invalid_argument("null argument");
+        *task_bytes = static_cast<int32_t>(job->job->taskBytes());
+        *result_bytes = static_cast<int32_t>(job->job->resultBytes());
+        *tasks = static_cast<int32_t>(job->job->taskCount());
+    });
+}
+
+int hss_host_job_next_task(hss_host_job *job, void *task, int32_t capacity, int32_t *length) {
+    return guarded([&] {
+        if (!job || !task || !length) throw std::invalid_argument("null argument");
+        if (capacity < static_cast<int32_t>(job->job->taskBytes()))
+            throw std::invalid_argument("task buffer too small");
+        std::vector<uint8_t> t;
+        *length = 0;
+        if (!job->job->nextTask(t)) return;
+        std::memcpy(task, t.data(), t.size());
+        *length = static_cast<int32_t>(t.size());
+    });
+}
+
+int hss_host_job_commit_task(hss_host_job *job, const void *result, int32_t length) {
+    return guarded([&] {
+        if (!job || length < 0) throw std::invalid_argument("null job or negative length");
+        job->job->commitTask(static_cast<const uint8_t *>(result), static_cast<size_t>(length));
+    });
+}
+
+int hss_host_job_commit(hss_host_job *job, char *dir, int32_t dir_capacity) {
+    return guarded([&] {
+        if (!job) throw std::invalid_argument("null job");
+        copy_path(job->job->commitJob(), dir, dir_capacity);
+    });
+}
+
+int hss_host_job_close(hss_host_job *job) {
+    return guarded([&] { delete job; });
+}
+
+int hss_host_worker_open(hss_host_job *job, int32_t device, hss_host_worker **worker) {
+    return guarded([&] {
+        if (!job || !worker || device < 0) throw std::invalid_argument("null argument or negative device");
+        *worker = nullptr;
+        auto w = std::make_unique<hss_host_worker>();
+        w->worker = std::make_unique<TaskWorker>(*job->job, static_cast<unsigned>(device));
+        *worker = w.release();
+    });
+}
+
+int hss_host_worker_run(hss_host_worker *worker, const void *task, int32_t length, void *result,
+                        int32_t capacity, int32_t *result_length) {
+    return guarded([&] {
+        if (!worker || !result || !result_length || length < 0)
+            throw std::invalid_argument("null argument or negative length");
+        std::vector<uint8_t> r;
+        worker->worker->run(static_cast<const uint8_t *>(task), static_cast<size_t>(length), r);
+        if (capacity < static_cast<int32_t>(r.size())) throw std::invalid_argument("result buffer too small");
+        std::memcpy(result, r.data(), r.size());
+        *result_length = static_cast<int32_t>(r.size());
+    });
+}
+
+int hss_host_worker_close(hss_host_worker *worker) {
+    return guarded([&] { delete worker; });
+}
 
 int hss_host_semblance_op(const hss_traces *traces, const hss_midpoints *mids, int model,
                           const float *d_points, const float *d_gradS, float *d_S,

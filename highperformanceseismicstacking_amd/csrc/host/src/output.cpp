@@ -121,3 +121,15 @@ void Dumper::dumpStatisticalResult(const std::string &name, const StatisticalMid
         out << kv.first << "," << r.getStatisticalResultForMidpoint(kv.first) << std::endl;
     }
 }
+
+void dumpResultSet(const Dumper &dumper, const std::string &inputPath, const Traveltime &model,
+                   const ResultSet &results) {
+    dumper.dumpGatherParameters(inputPath);
+    dumper.dumpTraveltime(&model);
+    for (unsigned int r = 0; r < model.getNumberOfResults(); r++)
+        dumper.dumpResult(model.getDescriptionForResult(r), results.getArrayForResult(r));
+    for (unsigned int i = 0; i < static_cast<unsigned int>(StatisticResult::CNT); i++) {
+        const StatisticResult r = static_cast<StatisticResult>(i);
+        dumper.dumpStatisticalResult(statisticName(r), results.get(r));
+    }
+}

@@ -103,18 +103,12 @@ void SingleHostRunner::runWorkers(unsigned int devices) {
 }
 
 void SingleHostRunner::writeResults(Dumper &dumper) {
-    dumper.dumpGatherParameters(parser->getInputFilePath());
-    dumper.dumpTraveltime(traveltime.get());
-    for (unsigned int r = 0; r < traveltime->getNumberOfResults(); r++) {
-        dumper.dumpResult(traveltime->getDescriptionForResult(r), resultSet->getArrayForResult(r));
-    }
+    dumpResultSet(dumper, parser->getInputFilePath(), *traveltime, *resultSet);
     std::ostringstream means;
     means << "statistics (mean over midpoints):";
     for (unsigned int i = 0; i < kStatisticCount; i++) {
         const StatisticResult r = static_cast<StatisticResult>(i);
-        const StatisticalMidpointResult &values = resultSet->get(r);
-        dumper.dumpStatisticalResult(statisticName(r), values);
-        means << ' ' << statisticName(r) << ' ' << values.getAverageOfAllMidpoints() << ';';
+        means << ' ' << statisticName(r) << ' ' << resultSet->get(r).getAverageOfAllMidpoints() << ';';
     }
     LOGI(means.str());
 }

@@ -62,6 +62,34 @@ int hss_host_parse_command_line(int program, int argc, const char **argv);
  * reference (it launches one midpoint at a time). */
 int hss_host_xcd_balanced_order(const int32_t *work, int32_t n, int32_t *order);
 
+/* ---- per-midpoint tasks --------------------------------------------------------------
+ * The reference's SPITS module as C calls (job manager: common/src/execution/
+ * SpitzJobManager.cpp:17-42, worker: SpitzWorker.cpp:10-40, committer: SpitzCommitter.cpp:
+ * 40-99), without the SPITS runtime.  A job is one command line of program 0
+ * (single_host_linear_search), 1 (single_host_de) or 2 (single_host_stretch_free): open
+ * parses it, reads the line and creates the output directory.  next_task hands out one
+ * task per CDP in ascending midpoint order (*length 0 when none is left); a task record is
+ * one float32 midpoint.  A worker (one device; hss_host_worker_* are the only calls here
+ * that need a GPU) turns a task into a result record: the midpoint, the midpoint's
+ * result arrays (results x ns float32, as the traveltime lists them) and its five
+ * statistics (float32, StatisticResult order).  commit_task stores a record (any order,
+ * any worker); commit writes the files the programs write and returns the directory.
+ * One job per process at a time (the line is process-wide, as the reference's Gather).
+ * record_sizes reports the two record sizes in bytes and the number of tasks. */
+typedef struct hss_host_job hss_host_job;
+typedef struct hss_host_worker hss_host_worker;
+int hss_host_job_open(int program, int argc, const char **argv, hss_host_job **job);
+int hss_host_job_record_sizes(const hss_host_job *job, int32_t *task_bytes,
+                              int32_t *result_bytes, int32_t *tasks);
+int hss_host_job_next_task(hss_host_job *job, void *task, int32_t capacity, int32_t *length);
+int hss_host_job_commit_task(hss_host_job *job, const void *result, int32_t length);
+int hss_host_job_commit(hss_host_job *job, char *dir, int32_t dir_capacity);
+int hss_host_job_close(hss_host_job *job);
+int hss_host_worker_open(hss_host_job *job, int32_t device, hss_host_worker **worker);
+int hss_host_worker_run(hss_host_worker *worker, const void *task, int32_t length,
+                        void *result, int32_t capacity, int32_t *result_length);
+int hss_host_worker_close(hss_host_worker *worker);
+
 /* ---- device calls (need a GPU) ------------------------------------------------------
  * SemblanceOp (host/include/semblance_op.hpp), the torch.autograd.Function-shaped C++ op
  * over the adjoint: forward(d_points [n][np][ns]) -> d_S [n][ns], then
