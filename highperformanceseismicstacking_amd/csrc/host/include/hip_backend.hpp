@@ -57,6 +57,8 @@ class MidpointBatch {
     hss_midpoints view(float h0, float apm) const;
     size_t size() const { return m0.size(); }
     int count(size_t g) const { return cnt[g]; }
+    // Keeps the midpoints at the given positions (ascending), in that order; upload() again.
+    void retain(const std::vector<size_t> &positions);
 
   private:
     std::vector<float> m0;

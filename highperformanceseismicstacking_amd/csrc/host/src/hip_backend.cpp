@@ -2,6 +2,7 @@
 #include "hip_backend.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -167,6 +168,25 @@ void MidpointBatch::upload(hipStream_t stream) {
     }
 }
 
+void MidpointBatch::retain(const std::vector<size_t> &positions) {
+    std::vector<float> m;
+    std::vector<int32_t> b, c, idx;
+    for (size_t p : positions) {
+        m.push_back(m0.at(p));
+        c.push_back(cnt.at(p));
+        if (indexed) {
+            b.push_back(static_cast<int32_t>(idx.size()));
+            idx.insert(idx.end(), index.begin() + beg[p], index.begin() + beg[p] + cnt[p]);
+        } else {
+            b.push_back(beg[p]);
+        }
+    }
+    m0 = std::move(m);
+    beg = std::move(b);
+    cnt = std::move(c);
+    index = std::move(idx);
+}
+
 hss_midpoints MidpointBatch::view(float h0, float apm) const {
     hss_midpoints v{};
     v.n = static_cast<int32_t>(m0.size());
@@ -205,6 +225,44 @@ static std::vector<std::vector<int32_t>> masks_to_lists(const DeviceBuffer &mask
         for (size_t i = 0; i < ntr; i++) {
             if (host[g * ntr + i]) lists[g].push_back(static_cast<int32_t>(i));
         }
+    }
+    return lists;
+}
+
+// OCT selection (hss_select_traces) of the trace lists of m0s, in chunks of midpoints: at
+// most 65535 per call (the selection kernel's grid limit) and about 256 MiB of per-trace
+// mask each (the environment variable HSS_SELECT_MASK_BYTES sets another budget), so a
+// whole line in one batch needs neither a mask of midpoints x traces bytes nor one call
+// past the limit.  `chunk(first, oct)` adapts the candidate description to the chunk
+// starting at midpoint `first` (per-sample parameters).
+static size_t select_mask_bytes() {
+    const char *v = std::getenv("HSS_SELECT_MASK_BYTES");
+    if (v && *v) {
+        const unsigned long long b = std::strtoull(v, nullptr, 10);
+        if (b > 0) return static_cast<size_t>(b);
+    }
+    return size_t(256) << 20;
+}
+
+template <class Chunk>
+static std::vector<std::vector<int32_t>> select_oct_lists(const HipLine &line, const std::vector<float> &m0s,
+                                                          float apm, float h0, hss_oct_candidates oct,
+                                                          DeviceBuffer &dMask, hipStream_t stream,
+                                                          Chunk &&chunk) {
+    const size_t ntr = std::max<size_t>(line.traces().ntr, 1);
+    const size_t per = std::max<size_t>(1, std::min<size_t>(65535, select_mask_bytes() / ntr));
+    std::vector<std::vector<int32_t>> lists;
+    lists.reserve(m0s.size());
+    DeviceBuffer dm0(std::max<size_t>(std::min(per, m0s.size()), 1) * 4);
+    for (size_t first = 0; first < m0s.size(); first += per) {
+        const size_t n = std::min(per, m0s.size() - first);
+        dm0.upload(m0s.data() + first, n * 4, stream);
+        dMask.resize(std::max<size_t>(n * ntr, 1));
+        hss_oct_candidates c = oct;
+        chunk(first, c);
+        HSS_ASSERT(hss_select_traces(&line.traces(), OCT, static_cast<int32_t>(n), dm0.as<float>(), apm,
+                                     h0, &c, dMask.as<uint8_t>(), stream));
+        for (auto &l : masks_to_lists(dMask, n, line.traces().ntr, stream)) lists.push_back(std::move(l));
     }
     return lists;
 }
@@ -326,19 +384,12 @@ void HipLinearSearchAlgorithm::selectBatch(const std::vector<float> &m0s) {
             batch.addRange(m0, r.begin, r.count);
         }
     } else {
-        const size_t ntr = line.traces().ntr;
-        DeviceBuffer dm0(std::max<size_t>(m0s.size(), 1) * 4);
-        dm0.upload(m0s.data(), m0s.size() * 4, stream());
-        dMask.resize(std::max<size_t>(m0s.size() * ntr, 1));
         hss_oct_candidates oct{};
         oct.mode = HSS_OCT_PAIRS;
         oct.d_pairs = dOctPairs.as<float>();
         oct.npairs = static_cast<int32_t>(dOctPairs.size() / (2 * sizeof(float)));
-        HSS_ASSERT(hss_select_traces(&line.traces(), model, static_cast<int32_t>(m0s.size()),
-                                     dm0.as<float>(), gather->getApm(),
-                                     traveltime->getReferenceHalfoffset(), &oct,
-                                     dMask.as<uint8_t>(), stream()));
-        auto lists = masks_to_lists(dMask, m0s.size(), ntr, stream());
+        auto lists = select_oct_lists(line, m0s, gather->getApm(), traveltime->getReferenceHalfoffset(), oct,
+                                      dMask, stream(), [](size_t, hss_oct_candidates &) {});
         for (size_t g = 0; g < m0s.size(); g++) batch.addIndexList(m0s[g], lists[g]);
     }
     batch.upload(stream());
@@ -390,7 +441,10 @@ void HipLinearSearchAlgorithm::computeSemblanceAndParametersForMidpoints(
     std::vector<int> counts(m0s.size());
     for (size_t g = 0; g < m0s.size(); g++) counts[g] = batch.count(g);
     // one launch for all of them (the batch carries the XCD-balanced launch order)
-    selectBatch(launchM0);
+    if (slot.size() != m0s.size()) {
+        batch.retain(slot);
+        batch.upload(stream());
+    }
     out.assign(m0s.size(), MidpointOutput{});
     std::vector<unsigned long long> nu(launchM0.size(), 0);
     std::vector<float> res(static_cast<size_t>(launchM0.size()) * nres * ns, 0.0f);
@@ -677,18 +731,15 @@ void HipStretchFreeAlgorithm::selectBatch(const std::vector<float> &m0s) {
         }
     } else {
         // the per-sample (v, slope) of each midpoint (oct/stretch_free.cu:99-138)
-        const size_t ntr = line.traces().ntr;
-        DeviceBuffer dm0(std::max<size_t>(m0s.size(), 1) * 4);
-        dm0.upload(m0s.data(), m0s.size() * 4, stream());
-        dMask.resize(std::max<size_t>(m0s.size() * ntr, 1));
         hss_oct_candidates oct{};
         oct.mode = HSS_OCT_PER_SAMPLE;
-        oct.d_vparams = dVparams.as<float>();
-        HSS_ASSERT(hss_select_traces(&line.traces(), model, static_cast<int32_t>(m0s.size()),
-                                     dm0.as<float>(), gather->getApm(),
-                                     traveltime->getReferenceHalfoffset(), &oct,
-                                     dMask.as<uint8_t>(), stream()));
-        auto lists = masks_to_lists(dMask, m0s.size(), ntr, stream());
+        const float *vparams = dVparams.as<float>();
+        const size_t perMidpoint = size_t(2) * line.traces().ns;  // [n][2][ns]
+        oct.d_vparams = vparams;
+        auto lists = select_oct_lists(line, m0s, gather->getApm(), traveltime->getReferenceHalfoffset(), oct,
+                                      dMask, stream(), [&](size_t first, hss_oct_candidates &c) {
+                                          c.d_vparams = vparams + first * perMidpoint;
+                                      });
         for (size_t g = 0; g < m0s.size(); g++) batch.addIndexList(m0s[g], lists[g]);
     }
     batch.upload(stream());

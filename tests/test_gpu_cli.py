@@ -17,10 +17,10 @@ pytestmark = pytest.mark.gpu
 f32 = np.float32
 
 
-def run(prog, args, tmp, timeout=120):
+def run(prog, args, tmp, timeout=120, env=None):
     exe = os.path.join(BIN_DIR, prog)
     res = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True,
-                         timeout=timeout, cwd=tmp)
+                         timeout=timeout, cwd=tmp, env=None if env is None else {**os.environ, **env})
     assert res.returncode == 0, res.stdout + res.stderr
     assert "Results written to" in res.stdout, res.stdout
     outdirs = [l.split("Results written to ", 1)[1].strip() for l in res.stdout.splitlines()
@@ -276,6 +276,44 @@ def test_linear_search_zocrs_and_oct(tmp_path):
                       h0=h0, apm=apm)
         rres, _, _, _, _ = O.grid_f32(O.OCT, og, params, full=False)
         assert np.array_equal(got.view(np.uint32), rres.view(np.uint32)), m
+
+
+def test_oct_selection_in_chunks_changes_nothing(tmp_path):
+    """The host layer selects OCT traces in chunks of midpoints under a mask budget
+    (HSS_SELECT_MASK_BYTES); chunks of three midpoints give the linear search and the
+    stretch-free re-stack (per-sample parameters, offset per chunk) the files of one
+    chunk for the whole line."""
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 12)
+    line = synth.Line(line.samples[:, :201].copy(), line.sx, line.gx, line.scalco, 201, line.dt_us,
+                      line.cdp)
+    path = str(tmp_path / "o.su")
+    synth.write_su(path, line)
+    lo, hi = [1500.0, -5e-4], [4500.0, 5e-4]
+    common = ["--aph", "5000", "--apm", 30.0, "--azimuth", "90", "--input", path, "--tau", "0.02",
+              "--traveltime", "oct", "--lower-bounds", *lo, "--upper-bounds", *hi, "--h0", 100.0,
+              "--variant", "exact"]
+    small = {"HSS_SELECT_MASK_BYTES": str(3 * line.ntr)}
+    dirs = {}
+    for name, env in (("one", None), ("chunks", small)):
+        os.makedirs(tmp_path / name / "ls", exist_ok=True)
+        os.makedirs(tmp_path / name / "sf", exist_ok=True)
+        ls = run("single_host_linear_search", common + ["--output", str(tmp_path / name / "ls"),
+                                                        "--granularity", 6, 4], tmp_path, env=env)
+        sf = run("single_host_stretch_free",
+                 common + ["--output", str(tmp_path / name / "sf"), "--stack-datafiles",
+                           os.path.join(ls[0], "velocity.su"), os.path.join(ls[0], "slope.su")],
+                 tmp_path, env=env)
+        dirs[name] = (ls[0], sf[0])
+    for i, files in enumerate((["coherence.su", "stack.su", "velocity.su", "slope.su",
+                                "selected_traces.csv"],
+                               ["coherence.su", "stack.su", "n.su", "selected_traces.csv"])):
+        for f in files:
+            a = open(os.path.join(dirs["one"][i], f), "rb").read()
+            b = open(os.path.join(dirs["chunks"][i], f), "rb").read()
+            assert a == b, f
+    _, coh = synth.read_su(os.path.join(dirs["one"][1], "coherence.su"), line.ns)
+    assert np.count_nonzero(coh) > 0
 
 
 def test_differential_evolution_oct_exact_and_fast(tmp_path):
