@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 peak (dual-issue / packed FP32 counted)
-FP32_FMA_ISSUE_TFLOPS = 78.6  # one FP32 FMA per lane per cycle: 256 CU x 128 lanes x 2 x 2.4 GHz
+FP32_FMA_ISSUE_TFLOPS = 78.6  # non-packed FP32 FMAs on 16-lane SIMDs: 256 CU x 4 x 16 x 2 x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 128 bytes per clock per CU
 
 
@@ -365,6 +365,15 @@ def grid_extras(args, gl, grid_ms):
     progress(f"FP16 step: grid {g16_ms:.1f} ms, stretch-free {sf16_ms:.1f} ms")
     out["f16_storage"] = {"grid_launch_ms": g16_ms, "stretch_free_launch_ms": sf16_ms,
                           "evals_per_s": (g16.grid_evals + g16.sf_evals) / ((g16_ms + sf16_ms) * 1e-3)}
+    # its LDS traffic (the wave segments), per launch from the committed counter summary,
+    # over this run's launch time
+    p16 = read_profile_figures("grid", gl.name, "f16")
+    if p16.get("lds_bytes_per_launch") is not None:
+        lds_gbs = p16["lds_bytes_per_launch"] / (g16_ms * 1e-3) / 1e9
+        out["f16_storage"].update({
+            "lds_gbs": lds_gbs, "lds_peak_gbs": LDS_PEAK_GBS, "lds_frac": lds_gbs / LDS_PEAK_GBS,
+            "hbm_traffic_bytes_per_launch": p16.get("traffic_bytes_per_launch"),
+            "counters_source": p16["source"]})
     # the precision path: FP16 storage, compensated window sums, then the adjoint polish
     # of every winner (Gauss-Newton steps on the semblance gradient)
     kah_ms = time_region(lambda: g16.grid(KAHAN, stream), 2, 1, stream)

@@ -176,9 +176,10 @@ int hss_grid_semblance_plain(const hss_traces *traces, int model, const float *d
  * 4 / t0_tiles candidate groups); both 0 when that key has not been tuned yet.  No
  * counterpart in the reference.  hss_grid_semblance tunes on its first launch of at
  * least 4096 workgroups for a key (outside stream capture) by timing each of the six
- * shapes on the first launch slots of the caller's batch (about 8192 workgroups); the
- * environment variable HSS_AUTOTUNE=0 turns it off.  Results do not depend on the shape
- * (bit for bit). */
+ * shapes on the first launch slots of the caller's batch (about 8192 workgroups) and
+ * keeping the fastest, a shape listed earlier (two candidates per lane, one t0 tile
+ * first) winning ties within 1%; the environment variable HSS_AUTOTUNE=0 turns it off.
+ * Results do not depend on the shape (bit for bit). */
 int hss_grid_tuned_shape(int model, int w, int variant, int storage, int32_t *cpl,
                          int32_t *t0_tiles);
 /* hss_grid_tuned_shape's candidates per lane only. */
