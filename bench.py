@@ -61,6 +61,8 @@ def parse():
                     help="pin the grid parameter tile (candidates per lane); 0 = autotune")
     ap.add_argument("--t0-tiles", type=int, default=1, choices=[1, 2, 4],
                     help="t0 tiles per workgroup when --cpl pins the shape")
+    ap.add_argument("--staged", type=int, default=0, choices=[0, 1],
+                    help="workgroup-staged traces (CMP, one t0 tile) when --cpl pins the shape")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step directly instead of replaying captured hipGraphs")
     ap.add_argument("--no-extras", action="store_true",
@@ -463,7 +465,7 @@ def run_grid(args):
     variant = {"fast": FAST, "exact": EXACT, "kahan": KAHAN}[args.variant]
     gl = GridLine(args.workload, storage)
     if args.cpl:
-        capi.grid_set_shape(CMP, gl.w, variant, storage, args.cpl, args.t0_tiles)
+        capi.grid_set_shape(CMP, gl.w, variant, storage, args.cpl, args.t0_tiles, args.staged)
     with_sf = args.workload == "c5"
     progress(f"{args.workload} line resident ({gl.mp.n} midpoints)")
     wall_ms, grid_ms, sf_ms = time_grid_steps(gl, variant, args.steps, args.warmup, with_sf,
@@ -481,7 +483,7 @@ def run_grid(args):
         "grid_launch_ms": grid_ms,
         "stretch_free_launch_ms": sf_ms,
         "graph": not args.no_graph,
-        "launch_shape": dict(zip(("candidates_per_lane", "t0_tiles_per_workgroup"),
+        "launch_shape": dict(zip(("candidates_per_lane", "t0_tiles_per_workgroup", "staged"),
                                  capi.grid_tuned_shape(CMP, gl.w, variant, storage))),
         "launch_order": "xcd_balanced",
     }

@@ -95,8 +95,8 @@ def lib():
         L.hss_grid_set_config.argtypes = [ci, ci, ci, ci, i32]
         L.hss_semblance_vjp.argtypes = [i32, i32, i32, vp, vp, vp, vp]
         L.hss_grid_tuned_config.argtypes = [ci, ci, ci, ci, ctypes.POINTER(i32)]
-        L.hss_grid_tuned_shape.argtypes = [ci, ci, ci, ci, ctypes.POINTER(i32), ctypes.POINTER(i32)]
-        L.hss_grid_set_shape.argtypes = [ci, ci, ci, ci, i32, i32]
+        L.hss_grid_tuned_shape.argtypes = [ci, ci, ci, ci] + [ctypes.POINTER(i32)] * 3
+        L.hss_grid_set_shape.argtypes = [ci, ci, ci, ci, i32, i32, i32]
         L.hss_polish.argtypes = [tr, mp, ci, vp, vp, i32, vp, vp, vp]
         L.hss_select_traces.argtypes = [tr, ci, i32, vp, f32, f32,
                                         ctypes.POINTER(hss_oct_candidates), vp, vp]
@@ -481,19 +481,20 @@ def grid_set_config(model, w, variant, storage, cpl):
     check(lib().hss_grid_set_config(int(model), int(w), int(variant), int(storage), int(cpl)))
 
 
-def grid_set_shape(model, w, variant, storage, cpl, t0_tiles):
-    """Pins the grid launch shape (candidates per lane, t0 tiles per workgroup); cpl 0
-    forgets the key."""
+def grid_set_shape(model, w, variant, storage, cpl, t0_tiles, staged=0):
+    """Pins the grid launch shape (candidates per lane, t0 tiles per workgroup, LDS-staged
+    traces); cpl 0 forgets the key."""
     check(lib().hss_grid_set_shape(int(model), int(w), int(variant), int(storage), int(cpl),
-                                   int(t0_tiles)))
+                                   int(t0_tiles), int(staged)))
 
 
 def grid_tuned_shape(model, w, variant, storage):
-    """(candidates per lane, t0 tiles per workgroup) chosen for a key ((0, 0): not tuned)."""
-    c, t = ctypes.c_int32(0), ctypes.c_int32(0)
+    """(candidates per lane, t0 tiles per workgroup, staged) chosen for a key ((0, 0, 0):
+    not tuned)."""
+    c, t, st = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
     check(lib().hss_grid_tuned_shape(int(model), int(w), int(variant), int(storage),
-                                     ctypes.byref(c), ctypes.byref(t)))
-    return int(c.value), int(t.value)
+                                     ctypes.byref(c), ctypes.byref(t), ctypes.byref(st)))
+    return int(c.value), int(t.value), int(st.value)
 
 
 def grid_tuned_config(model, w, variant, storage):

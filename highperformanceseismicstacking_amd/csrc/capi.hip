@@ -6,6 +6,7 @@
 #include "kernels/adjoint.hpp"
 #include "kernels/de_impl.hpp"
 #include "kernels/grid_impl.hpp"
+#include "kernels/grid_staged.hpp"
 #include "kernels/launch_util.hpp"
 #include "kernels/sf_impl.hpp"
 
@@ -191,7 +192,10 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
         la.tt = sh.tt;
         la.wg_per_mid = (a.ntiles + sh.tt - 1) / sh.tt;
         switch (model) {
-            case HSS_CMP: return launch_grid_cmp(la, slots, variant, traces->storage, sh.cpl, st);
+            case HSS_CMP:
+                if (sh.staged)
+                    return launch_grid_cmp_staged(la, slots, variant, traces->storage, sh.cpl, st);
+                return launch_grid_cmp(la, slots, variant, traces->storage, sh.cpl, st);
             case HSS_ZOCRS: return launch_grid_zocrs(la, slots, variant, traces->storage, sh.cpl, st);
             default: return launch_grid_oct(la, slots, variant, traces->storage, sh.cpl, st);
         }
@@ -199,7 +203,7 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     // timing runs must not add to the accumulating rejection counts
     GridArgs probe = a;
     probe.not_used = nullptr;
-    GridShape shape{1, 1};
+    GridShape shape{1, 1, 0};
     rc = tuned_grid_shape(model, traces->w, variant, traces->storage, mids->n, a.ntiles, st,
                           [&](GridShape sh, int slots) { return run(probe, sh, slots); }, &shape);
     if (rc != HSS_OK) return rc;

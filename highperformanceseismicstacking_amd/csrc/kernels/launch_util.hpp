@@ -45,11 +45,13 @@ inline int hip_status(hipError_t e) { return e == hipSuccess ? HSS_OK : (int)e; 
 // first time).
 bool fast_index_verified(float dt, int tau, hipStream_t stream);
 
-// Grid-search launch shape: candidates per lane (cpl, 1 or 2) and 64-sample t0 tiles per
-// workgroup (tt, 1, 2 or 4).  Results do not depend on it.
+// Grid-search launch shape: candidates per lane (cpl, 1 or 2), 64-sample t0 tiles per
+// workgroup (tt, 1, 2 or 4) and, for CMP with tt = 1, traces staged in LDS for the whole
+// workgroup (staged, grid_staged.hpp).  Results do not depend on it.
 struct GridShape {
     int cpl;
     int tt;
+    int staged;
 };
 
 // The shape for this key, tuned on the first large launch by timing launch(shape, slots)

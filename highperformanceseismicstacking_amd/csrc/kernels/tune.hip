@@ -1,13 +1,16 @@
 // The launch-shape autotuner of the grid search.
 //
-// Two knobs (grid_impl.hpp), neither of which changes a result bit:
+// Three knobs (grid_impl.hpp, grid_staged.hpp), none of which changes a result bit:
 //  * the parameter tile, candidates per lane (CPL = 1 or 2): two candidates share the
 //    trace loop, the geometry broadcast and (16-bit storage) the LDS segment passes, at
 //    the price of a second set of window registers;
 //  * the t0 tile, 64-sample tiles per workgroup (TT = 1, 2 or 4): the workgroup's four
 //    waves split into TT tiles x 4/TT candidate groups, trading the number of workgroups
 //    per midpoint (and so how a midpoint's work spreads over the CUs of its XCD) against
-//    the candidates each wave walks.
+//    the candidates each wave walks;
+//  * staging (CMP, one t0 tile per workgroup): the four waves walk the traces together and
+//    read their windows from the workgroup's LDS copy of each trace's span, instead of
+//    each lane (FP32) or each wave (16-bit) reading its own.
 // Which shape wins depends on the model, the window size, the storage and the variant.
 // The first launch for a key (device, model, w, variant, storage) that is large enough
 // to time reliably runs every shape on a leading part of the caller's own batch (its
@@ -44,7 +47,8 @@ bool tuning_enabled() {
 // first -- two candidates per lane first, whose four resident workgroups per CU keep a
 // midpoint's traces in the XCD's L2 (one candidate per lane, at six, can fetch many times
 // the line past it for the same time).
-constexpr GridShape kShapes[] = {{2, 1}, {1, 1}, {2, 2}, {1, 2}, {2, 4}, {1, 4}};
+constexpr GridShape kShapes[] = {{2, 1, 0}, {1, 1, 0}, {2, 2, 0}, {1, 2, 0},
+                                 {2, 4, 0}, {1, 4, 0}, {2, 1, 1}, {1, 1, 1}};
 constexpr float kTuneMargin = 0.01f;
 // workgroups a timing run covers (a leading part of the batch); the warm-up run before it
 // (code load) covers a sixteenth of that
@@ -65,7 +69,7 @@ int tuned_grid_shape(int model, int w, int variant, int storage, int n_slots, in
             return HSS_OK;
         }
     }
-    *shape_out = GridShape{1, 1};
+    *shape_out = GridShape{1, 1, 0};
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return HSS_OK;
@@ -85,6 +89,7 @@ int tuned_grid_shape(int model, int w, int variant, int storage, int n_slots, in
     bool timed = false;
     int rc = HSS_OK;
     for (const GridShape &sh : kShapes) {
+        if (sh.staged && model != HSS_CMP) continue;
         rc = launch(sh, std::max(1, slots / 16));  // warm-up (code load)
         if (rc != HSS_OK) break;
         if (hipEventRecord(e0, stream) != hipSuccess) break;
@@ -121,40 +126,44 @@ void tuned_grid_store(int dev, int model, int w, int variant, int storage, GridS
 GridShape tuned_grid_lookup(int dev, int model, int w, int variant, int storage) {
     std::lock_guard<std::mutex> lock(tune_mu);
     auto it = tune_cache.find(std::make_tuple(dev, model, w, variant, storage));
-    return it == tune_cache.end() ? GridShape{0, 0} : it->second;
+    return it == tune_cache.end() ? GridShape{0, 0, 0} : it->second;
 }
 
 }  // namespace hss
 
-// The launch shape the autotuner chose for a key on the current device ({0, 0}: not yet).
+// The launch shape the autotuner chose for a key on the current device (all 0: not yet).
 extern "C" int hss_grid_tuned_shape(int model, int w, int variant, int storage, int32_t *cpl,
-                                    int32_t *t0_tiles) {
-    if (!cpl || !t0_tiles) return HSS_ERR_NULL_POINTER;
+                                    int32_t *t0_tiles, int32_t *staged) {
+    if (!cpl || !t0_tiles || !staged) return HSS_ERR_NULL_POINTER;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return HSS_ERR_NO_DEVICE;
     const hss::GridShape sh = hss::tuned_grid_lookup(dev, model, w, variant, storage);
     *cpl = sh.cpl;
     *t0_tiles = sh.tt;
+    *staged = sh.staged;
     return HSS_OK;
 }
 
 extern "C" int hss_grid_tuned_config(int model, int w, int variant, int storage, int32_t *cpl) {
-    int32_t tt = 0;
-    return hss_grid_tuned_shape(model, w, variant, storage, cpl, &tt);
+    int32_t tt = 0, staged = 0;
+    return hss_grid_tuned_shape(model, w, variant, storage, cpl, &tt, &staged);
 }
 
 // Pins the launch shape for a key on the current device; cpl = 0 forgets the key (the
 // next large launch tunes it again).
 extern "C" int hss_grid_set_shape(int model, int w, int variant, int storage, int32_t cpl,
-                                  int32_t t0_tiles) {
+                                  int32_t t0_tiles, int32_t staged) {
     if (cpl < 0 || cpl > 2) return HSS_ERR_INVALID_SIZE;
     if (cpl > 0 && t0_tiles != 1 && t0_tiles != 2 && t0_tiles != 4) return HSS_ERR_INVALID_SIZE;
+    if (staged != 0 && staged != 1) return HSS_ERR_INVALID_SIZE;
+    if (cpl > 0 && staged && t0_tiles != 1) return HSS_ERR_INVALID_SIZE;
+    if (cpl > 0 && staged && model != HSS_CMP) return HSS_ERR_INVALID_MODEL;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return HSS_ERR_NO_DEVICE;
-    hss::tuned_grid_store(dev, model, w, variant, storage, hss::GridShape{cpl, t0_tiles});
+    hss::tuned_grid_store(dev, model, w, variant, storage, hss::GridShape{cpl, t0_tiles, staged});
     return HSS_OK;
 }
 
 extern "C" int hss_grid_set_config(int model, int w, int variant, int storage, int32_t cpl) {
-    return hss_grid_set_shape(model, w, variant, storage, cpl, 1);
+    return hss_grid_set_shape(model, w, variant, storage, cpl, 1, 0);
 }

@@ -173,23 +173,27 @@ int hss_grid_semblance_plain(const hss_traces *traces, int model, const float *d
 /* The launch shape the grid-search autotuner chose on the current device for (model, w,
  * variant, storage): *cpl = candidates per lane (1 or 2), *t0_tiles = 64-sample t0 tiles
  * per workgroup (1, 2 or 4; the workgroup's four waves split into that many tiles times
- * 4 / t0_tiles candidate groups); both 0 when that key has not been tuned yet.  No
+ * 4 / t0_tiles candidate groups), *staged = 1 when the workgroup stages each trace's span
+ * in LDS for all its waves (CMP, one t0 tile per workgroup); all 0 when that key has not
+ * been tuned yet.  No
  * counterpart in the reference.  hss_grid_semblance tunes on its first launch of at
- * least 4096 workgroups for a key (outside stream capture) by timing each of the six
- * shapes on the first launch slots of the caller's batch (about 8192 workgroups) and
- * keeping the fastest, a shape listed earlier (two candidates per lane, one t0 tile
- * first) winning ties within 1%; the environment variable HSS_AUTOTUNE=0 turns it off.
- * Results do not depend on the shape (bit for bit). */
+ * least 4096 workgroups for a key (outside stream capture) by timing each shape (six
+ * unstaged, and for CMP two staged) on the first launch slots of the caller's batch
+ * (about 8192 workgroups) and keeping the fastest, a shape listed earlier (two candidates
+ * per lane, one t0 tile, unstaged first) winning ties within 1%; the environment variable
+ * HSS_AUTOTUNE=0 turns it off.  Results do not depend on the shape (bit for bit). */
 int hss_grid_tuned_shape(int model, int w, int variant, int storage, int32_t *cpl,
-                         int32_t *t0_tiles);
+                         int32_t *t0_tiles, int32_t *staged);
 /* hss_grid_tuned_shape's candidates per lane only. */
 int hss_grid_tuned_config(int model, int w, int variant, int storage, int32_t *cpl);
 
 /* Pins the grid-search launch shape for (model, w, variant, storage) on the current
- * device: cpl = 1 or 2 candidates per lane, t0_tiles = 1, 2 or 4; cpl = 0 forgets the key
- * (the next large launch tunes it again).  hss_grid_set_config(.., cpl) pins (cpl, 1). */
+ * device: cpl = 1 or 2 candidates per lane, t0_tiles = 1, 2 or 4, staged = 0 or 1 (1 only
+ * for CMP with t0_tiles = 1: HSS_ERR_INVALID_MODEL / HSS_ERR_INVALID_SIZE otherwise);
+ * cpl = 0 forgets the key (the next large launch tunes it again).
+ * hss_grid_set_config(.., cpl) pins (cpl, 1, 0). */
 int hss_grid_set_shape(int model, int w, int variant, int storage, int32_t cpl,
-                       int32_t t0_tiles);
+                       int32_t t0_tiles, int32_t staged);
 int hss_grid_set_config(int model, int w, int variant, int storage, int32_t cpl);
 
 /* Bytes of device workspace hss_de_semblance needs for n_mid midpoints of ns samples

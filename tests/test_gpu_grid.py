@@ -181,13 +181,15 @@ def test_plain_per_gather_kernel_equals_batched_exact(model):
     assert np.array_equal(plain[0].view(np.uint32), rres.view(np.uint32))
 
 
-@pytest.mark.parametrize("storage", [F32, BF16])
+@pytest.mark.parametrize("storage", [F32, F16, BF16])
 @pytest.mark.parametrize("variant", [EXACT, FAST, KAHAN])
 def test_launch_shape_does_not_change_results(storage, variant):
     """Every launch shape of the autotuner (1 or 2 candidates per lane, 1, 2 or 4 t0 tiles
-    per workgroup) gives the one-candidate, one-tile results bit for bit, including an odd
-    candidate count (a half-empty last parameter tile) and 13 t0 tiles (workgroups whose
-    last waves have no tile)."""
+    per workgroup, and the workgroup-staged traces) gives the one-candidate, one-tile
+    results bit for bit, including an odd candidate count (a half-empty last parameter
+    tile and last staging round), 13 t0 tiles (workgroups whose last waves have no tile)
+    and rounds whose candidates spread the far traces' windows over more samples than the
+    stage holds (those traces take the direct reads)."""
     import torch
     line = synth.make_line(5, (30, 60), 3000.0, 801, 2000, seed=21, offset_min=50.0)
     tau = synth.tau_index(0.011, line.dt_us)
@@ -197,18 +199,24 @@ def test_launch_shape_does_not_change_results(storage, variant):
     w = 2 * tau + 1
     out = {}
     assert (line.ns + 63) // 64 == 13
+    shapes = [(cpl, tt, 0) for cpl in (1, 2) for tt in (1, 2, 4)] + [(1, 1, 1), (2, 1, 1)]
     try:
-        for cpl in (1, 2):
-            for tt in (1, 2, 4):
-                capi.grid_set_shape(CMP, w, variant, storage, cpl, tt)
-                assert capi.grid_tuned_shape(CMP, w, variant, storage) == (cpl, tt)
-                out[(cpl, tt)] = run_gpu(line, b, c, m0, CMP, params, tau, variant,
-                                         storage=storage)
+        for shape in shapes:
+            capi.grid_set_shape(CMP, w, variant, storage, *shape)
+            assert capi.grid_tuned_shape(CMP, w, variant, storage) == shape
+            out[shape] = run_gpu(line, b, c, m0, CMP, params, tau, variant, storage=storage)
     finally:
-        capi.grid_set_shape(CMP, w, variant, storage, 0, 0)
+        capi.grid_set_shape(CMP, w, variant, storage, 0, 0, 0)
     for key, r in out.items():
         for i in range(3):
-            assert np.array_equal(out[(1, 1)][i].view(np.uint32), r[i].view(np.uint32)), (key, i)
+            assert np.array_equal(out[(1, 1, 0)][i].view(np.uint32), r[i].view(np.uint32)), (key, i)
+
+
+def test_staged_shape_is_refused_where_it_does_not_apply():
+    with pytest.raises(capi.HssError):
+        capi.grid_set_shape(ZOCRS, 11, FAST, F32, 2, 1, 1)
+    with pytest.raises(capi.HssError):
+        capi.grid_set_shape(CMP, 11, FAST, F32, 2, 2, 1)
 
 
 def test_line_batch_equals_single_midpoints():
