@@ -24,13 +24,25 @@
 
 namespace hss {
 
-constexpr int kStageCap = 256;  // samples staged per trace
-static_assert(kStageCap == kGridThreads, "one staged sample per thread and trace");
-// Traces per stage: 32 (32 KB of LDS, four workgroups per CU, as many as the registers of
-// two candidates per lane allow; 4% faster than 16 on the C5 line), and for HSS_KAHAN its
-// compensation chunk, since its sums are flushed at those fixed trace blocks.
+// Samples staged per trace: 128, loaded by the first two waves (one sample per thread).
+// The C5 line's spans (a 64-sample t0 tile plus the window and the moveout across the
+// round's candidates) fit; a 256-sample stage was 4.7% slower (more LDS per workgroup, one
+// workgroup fewer per CU).
+constexpr int kStageCap = 128;
+static_assert(kStageCap % kWave == 0 && kStageCap <= kGridThreads, "whole waves load the stage");
+// Traces per stage: 32 (16 KB of LDS; 48 was no faster, 16 was 3% slower on the C5 line),
+// and for HSS_KAHAN its compensation chunk, since its sums are flushed at those fixed
+// trace blocks.
 template <int VARIANT>
 constexpr int stage_traces() { return VARIANT == HSS_KAHAN ? kKahanChunk : 32; }
+
+// Register budget: five waves per SIMD (96 VGPRs, a few spilled), 4% faster than the four
+// the compiler picks by itself on the C5 line and faster than six; HSS_KAHAN keeps the
+// grid kernel's budget.
+template <int VARIANT, int CPL>
+constexpr int staged_min_waves() {
+    return grid_min_waves<VARIANT, CPL>() > 5 ? grid_min_waves<VARIANT, CPL>() : 5;
+}
 
 // The sample index (int)(t/dt) of the index chain, clamped into [-1, 2^24] first (the
 // span only needs the order; lanes use indices inside (lo, hi) of index_bounds()).
@@ -45,7 +57,7 @@ __device__ __forceinline__ int span_index(float t, float dt, float inv) {
 
 template <int W, int VARIANT, int STORAGE, int CPL>
 __global__ __launch_bounds__(kGridThreads)
-__attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void grid_staged_kernel(
+__attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void grid_staged_kernel(
     GridArgs a) {
     using Reader = SampleReader<STORAGE>;
     constexpr bool FASTIDX = W != 0;
@@ -132,9 +144,26 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
             const unsigned long long want =
                 nj >= kWave ? ~0ull : ((1ull << nj) - 1ull);
             const bool staged = (__builtin_amdgcn_ballot_w64(fit_l != 0) & want) == want;
-            if (staged) {
-                // all of the block's loads first, then the LDS writes (one wait, not one per
-                // trace)
+            if (staged && STORAGE == HSS_F32) {
+                // FP32: the samples go from global memory straight into LDS
+                // (global_load_lds_dword: lane l of wave v < kStageCap/64 writes
+                // stage[jj][64 v + l]), no registers in between; a lane past the trace's end
+                // loads nothing (its slot is never read)
+                __syncthreads();  // every wave has read the previous staged block
+                for (int jj = 0; jj < nj; ++jj) {
+                    const int base = __builtin_amdgcn_readlane(base_l, jj);
+                    const int ti = __builtin_amdgcn_readlane(ti_l, jj);
+                    const int i = base + (int)threadIdx.x;
+                    if (i < ns && wave < kStageCap / kWave)
+                        __builtin_amdgcn_global_load_lds(
+                            (const float *)e.samples + (size_t)ti * (size_t)ns + (size_t)i,
+                            (__attribute__((address_space(3))) void *)&stage[jj][wave * kWave], 4,
+                            0, 0);
+                }
+                __syncthreads();  // (the compiler waits for the loads before the barrier)
+            } else if (staged) {
+                // 16-bit: all of the block's loads first, then the decoded and scaled values
+                // into LDS (one wait, not one per trace)
                 float vals[kStageTraces];
 #pragma unroll
                 for (int jj = 0; jj < kStageTraces; ++jj) {
@@ -143,7 +172,7 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
                     const int base = __builtin_amdgcn_readlane(base_l, jj);
                     const int ti = __builtin_amdgcn_readlane(ti_l, jj);
                     const int i = base + (int)threadIdx.x;  // within the trace: no window reads past ns
-                    if (i < ns) vals[jj] = rd((size_t)ti * (size_t)ns + (size_t)i);
+                    if (i < ns && (int)threadIdx.x < kStageCap) vals[jj] = rd((size_t)ti * (size_t)ns + (size_t)i);
                 }
                 __syncthreads();  // every wave has read the previous staged block
 #pragma unroll
@@ -152,7 +181,7 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
                     float v = vals[jj];
                     // as the wave segments do (a sample past the trace stays 0 and is never read)
                     if (STORAGE != HSS_F32 || scaled) v = v * bcast(sc_l, jj);
-                    stage[jj][threadIdx.x] = v;
+                    if ((int)threadIdx.x < kStageCap) stage[jj][threadIdx.x] = v;
                 }
                 __syncthreads();
             }
