@@ -167,6 +167,9 @@ int hss_grid_semblance(const hss_traces *traces, const hss_midpoints *mids, int 
     if (rc != HSS_OK) return rc;
     if (!d_params || !d_result) return HSS_ERR_NULL_POINTER;
     if (P <= 0) return HSS_ERR_INVALID_SIZE;
+    // the kernels address a midpoint's [ns][P] slab of the cubes with 32-bit offsets
+    if ((d_sembl || d_stack) && (uint64_t)traces->ns * (uint64_t)P > 0xffffffffULL)
+        return HSS_ERR_INVALID_SIZE;
     if (mids->n == 0) return HSS_OK;
     GridArgs a{};
     a.e = base_ctx(traces, mids);

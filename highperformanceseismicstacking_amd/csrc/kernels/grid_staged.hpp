@@ -36,12 +36,15 @@ static_assert(kStageCap % kWave == 0 && kStageCap <= kGridThreads, "whole waves 
 template <int VARIANT>
 constexpr int stage_traces() { return VARIANT == HSS_KAHAN ? kKahanChunk : 32; }
 
-// Register budget: five waves per SIMD (96 VGPRs, a few spilled), 4% faster than the four
-// the compiler picks by itself on the C5 line and faster than six; HSS_KAHAN keeps the
-// grid kernel's budget.
+// Register budget: five waves per SIMD (96 VGPRs), 4% faster than the four the compiler
+// picks by itself on the C5 line and faster than six; HSS_KAHAN keeps the grid kernel's
+// budget.  At 96 registers nothing may be spilled: a spill slot is per-lane memory that
+// stays in L2 beside the traces (5 resident midpoints per XCD at five waves), and the C5
+// launch then read 63 GB and wrote 46 GB instead of about 3 GB.  Hence the running best in
+// LDS, the uniform values in scalar registers and thread_index_here().
 template <int VARIANT, int CPL>
 constexpr int staged_min_waves() {
-    return grid_min_waves<VARIANT, CPL>() > 5 ? grid_min_waves<VARIANT, CPL>() : 5;
+    return VARIANT == HSS_KAHAN ? grid_min_waves<VARIANT, CPL>() : 5;
 }
 
 // The sample index (int)(t/dt) of the index chain, clamped into [-1, 2^24] first (the
@@ -55,6 +58,32 @@ __device__ __forceinline__ int span_index(float t, float dt, float inv) {
     return (int)ti;
 }
 
+// A wave-uniform float held in a scalar register: readfirstlane, then an empty asm
+// statement with an "s" operand (readfirstlane alone is folded away when the compiler sees
+// that the value is uniform, and the value goes back to a vector register).
+__device__ __forceinline__ float uniform_f(float x) {
+    int b = __builtin_amdgcn_readfirstlane(__float_as_int(x));
+    __asm__("" : "+s"(b));
+    return __int_as_float(b);
+}
+
+// threadIdx.x through an empty asm statement: what is derived from it at a point of use
+// (the lane's LDS slots, its cube row, its output sample) is recomputed there from the one
+// register that holds the thread index, instead of being kept in registers of its own (or
+// spilled) across the trace loops.
+__device__ __forceinline__ int thread_index_here() {
+    int t = (int)threadIdx.x;
+    __asm__ volatile("" : "+v"(t));
+    return t;
+}
+
+// The running best of each thread, in LDS.
+struct RunBest {
+    float best[kGridThreads];
+    float stack[kGridThreads];
+    int idx[kGridThreads];
+};
+
 template <int W, int VARIANT, int STORAGE, int CPL>
 __global__ __launch_bounds__(kGridThreads)
 __attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void grid_staged_kernel(
@@ -62,7 +91,10 @@ __attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void g
     using Reader = SampleReader<STORAGE>;
     constexpr bool FASTIDX = W != 0;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    // wave-uniform values go through readfirstlane so that they live in scalar registers
+    // (the round's candidates are then scalar loads; gfx950 has no scalar float ALU, so a
+    // uniform float otherwise takes a vector register for the whole kernel)
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const int blk = logical_block(blockIdx.x, gridDim.x);
     const int slot = blk / a.ntiles;
     const int tile = blk - slot * a.ntiles;
@@ -77,15 +109,23 @@ __attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void g
     const bool scaled = (STORAGE != HSS_F32) && e.scale != nullptr;
     Reader rd{(decltype(Reader::p))e.samples};
     // the tile's first and last t0 (the span's corners)
-    const float t0a = (float)(tile * kWave) * e.dt;
-    const float t0b = (float)min(tile * kWave + kWave - 1, ns - 1) * e.dt;
-    const float t0a_sq = t0a * t0a, t0b_sq = t0b * t0b;
+    const float t0a = uniform_f((float)(tile * kWave) * e.dt);
+    const float t0b = uniform_f((float)min(tile * kWave + kWave - 1, ns - 1) * e.dt);
+    const float t0a_sq = uniform_f(t0a * t0a), t0b_sq = uniform_f(t0b * t0b);
+    // the midpoint's [ns][P] slabs of the optional cubes (the host keeps ns * P below 2^32)
+    float *const sembl_g = a.sembl ? a.sembl + (size_t)g * ns * a.P : nullptr;
+    float *const stack_g = a.stack ? a.stack + (size_t)g * ns * a.P : nullptr;
 
     constexpr int kStageTraces = stage_traces<VARIANT>();
     static_assert(kStageTraces <= kWave, "a block's traces are one wave's lanes");
     __shared__ float stage[kStageTraces][kStageCap];
-    float best = -1.0f, bstack = 0.0f;
-    int bidx = -1;
+    // the lane's running best (first strict maximum over its candidates in order), kept in
+    // LDS rather than in registers, which the window sums of two candidates use up; each
+    // thread reads and writes only its own slots
+    __shared__ RunBest run;
+    run.best[threadIdx.x] = -1.0f;
+    run.stack[threadIdx.x] = 0.0f;
+    run.idx[threadIdx.x] = -1;
 
     const int per_round = kGridWaves * CPL;
     const int rounds = (a.P + per_round - 1) / per_round;
@@ -113,8 +153,9 @@ __attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void g
         for (int j0 = 0; j0 < e.cnt; j0 += kStageTraces) {
             const int nj = min(kStageTraces, e.cnt - j0);
             // lane jj < nj: trace j0 + jj's geometry, scale and staged span
-            const bool have = lane < nj;
-            const int ti_l = have ? (e.midx ? e.midx[e.beg + j0 + lane] : e.beg + j0 + lane) : 0;
+            const int ln = thread_index_here() & (kWave - 1);
+            const bool have = ln < nj;
+            const int ti_l = have ? (e.midx ? e.midx[e.beg + j0 + ln] : e.beg + j0 + ln) : 0;
             const float h2_l = have ? e.hoff2[ti_l] : 0.0f;
             const float sc_l = (scaled && have) ? e.scale[ti_l] : 1.0f;
             int base_l = 0, fit_l = 0;
@@ -242,29 +283,45 @@ __attribute__((amdgpu_waves_per_eu(staged_min_waves<VARIANT, CPL>(), 8))) void g
 #pragma unroll
         for (int c = 0; c < CPL; ++c) acc[c].reduce(e.w, &S[c], &stk[c]);
         if (active) {
+            const int t = thread_index_here();
+            const uint32_t cube_row = (uint32_t)(tile * kWave + (t & (kWave - 1))) * (uint32_t)a.P;
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 if (!on[c]) continue;
                 const int q = q0 + c;
-                if (a.sembl) a.sembl[((size_t)g * ns + s) * a.P + q] = S[c];
-                if (a.stack) a.stack[((size_t)g * ns + s) * a.P + q] = stk[c];
-                if (S[c] > best) {
-                    best = S[c];
-                    bstack = stk[c];
-                    bidx = q;
+                if (sembl_g) sembl_g[cube_row + (uint32_t)q] = S[c];
+                if (stack_g) stack_g[cube_row + (uint32_t)q] = stk[c];
+                if (S[c] > run.best[t]) {
+                    run.best[t] = S[c];
+                    run.stack[t] = stk[c];
+                    run.idx[t] = q;
                 }
             }
         }
     }
 
-    float B, BS;
-    int BI;
-    merge_bests<kGridWaves>(best, bstack, bidx, wave, lane, 1, &B, &BS, &BI);
+    // the four waves' bests of each t0, as merge_bests merges them (the largest value, the
+    // smallest candidate index among equal values)
+    __syncthreads();
     if (wave != 0 || !active) return;
+    const int ln = thread_index_here() & (kWave - 1);
+    float B = -1.0f, BS = 0.0f;
+    int BI = -1;
+#pragma unroll
+    for (int v = 0; v < kGridWaves; ++v) {
+        const int iv = run.idx[v * kWave + ln];
+        const float sv = run.best[v * kWave + ln];
+        if (iv >= 0 && (sv > B || (sv == B && iv < BI))) {
+            B = sv;
+            BS = run.stack[v * kWave + ln];
+            BI = iv;
+        }
+    }
+    const int so = tile * kWave + ln;  // the lane's sample
     float *res = a.result + (size_t)g * 3 * ns;
-    res[s] = B;
-    res[ns + s] = BI >= 0 ? BS : 0.0f;
-    res[2 * ns + s] = BI >= 0 ? 2 / sqrtf(load_candidate<HSS_CMP>(a.params, BI).c) : 0.0f;
+    res[so] = B;
+    res[ns + so] = BI >= 0 ? BS : 0.0f;
+    res[2 * ns + so] = BI >= 0 ? 2 / sqrtf(load_candidate<HSS_CMP>(a.params, BI).c) : 0.0f;
 }
 
 // Host launcher: one workgroup per (launch slot, t0 tile) (a.tt must be 1).

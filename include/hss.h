@@ -146,7 +146,8 @@ int hss_build_parameter_array(int model, const float *lower, const float *step,
  * strict maximum over p as selectBestSemblancesFor<Model> does.
  *   d_params  [P][np] storage form (hss_build_parameter_array);
  *   d_result  [n][(2+np)][ns]: S, stack, v = 2/sqrt(c), then A,B / slope;
- *   d_sembl, d_stack [n][ns][P] every evaluated value (NULL to skip);
+ *   d_sembl, d_stack [n][ns][P] every evaluated value (NULL to skip; ns * P must stay
+ *             below 2^32, else HSS_ERR_INVALID_SIZE);
  *   d_not_used [n] OCT: number of (s,p,trace) triples the model rejected
  *             (NULL to skip; accumulated, caller zeroes it).
  * Replaces the launches in CudaLinearSearchAlgorithm::computeSemblanceAtGpuForMidpoint
