@@ -148,7 +148,13 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     constexpr int D = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     constexpr int REC = D + 2;  // D parameters, S, stack
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    // The wave index is uniform (readfirstlane), so the individual loops, their bounds and
+    // the donors' rows are scalar work.  What outlives an evaluation -- the trials, the
+    // lane's workspace column, its sample -- is kept in LDS or recomputed from the thread
+    // index where it is used (thread_index_here), so that the registers of the budget go
+    // to the trace walk and nothing is spilled: a spill slot is per-lane memory that takes
+    // L2 from the traces (this kernel waits on its trace reads).
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const int blk = logical_block(blockIdx.x, gridDim.x);
     const int slot = blk / a.ntiles;
     const int tile = blk - slot * a.ntiles;
@@ -162,23 +168,34 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     const EvalCtx e = midpoint_ctx(a.e, a.mbeg, a.mcnt, a.mm0, g);
     const uint32_t stream = a.stream_base + (uint32_t)g;
     const int NP = a.NP;
-    float *base = a.ws + ((size_t)g * a.ntiles + tile) * 2 * NP * REC * kWave;
+    float *const base = a.ws + ((size_t)g * a.ntiles + tile) * 2 * NP * REC * kWave;
     auto rec = [&](int buf, int i, int f) -> float * {
-        return base + ((size_t)(buf * NP + i) * REC + f) * kWave + lane;
+        return base + ((size_t)(buf * NP + i) * REC + f) * kWave +
+               (thread_index_here() & (kWave - 1));
     };
+    // the lane's sample, as the counter of its random stream
+    auto sample_here = [&]() { return (uint32_t)(tile * kWave + (thread_index_here() & (kWave - 1))); };
 
     __shared__ float seg_lds[kDeWaves][kSegFloats];
     __shared__ OctTraceRec trace_recs[kDeWaves][kWave];
-    unsigned long long rejected = 0;
     constexpr int CPL = de_cpl<MODEL, W, VARIANT>();
+    // the individuals a lane evaluates together, from their drawing to their selection
+    __shared__ float held[CPL * D][kDeWaves * kWave];
+    uint64_t rejected = 0;  // the wave's rejections (uniform)
     auto eval = [&](const Candidate (&kc)[CPL], const bool (&on)[CPL], float (&S)[CPL],
-                    float (&stk)[CPL], unsigned *rej) {
+                    float (&stk)[CPL]) {
+        unsigned rej = 0;  // on the wave's lowest active lane
         if constexpr (de_compact<MODEL, VARIANT>())
             evaluate_oct_compact<W, VARIANT, STORAGE, false, CPL>(
-                e, t0, kc, on, 0.0f, active, trace_recs[wave], S, stk, rej);
+                e, t0, kc, on, 0.0f, active, trace_recs[wave], S, stk, &rej);
         else
             evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep>(
-                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, rej);
+                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
+        if constexpr (MODEL == HSS_OCT) {
+            // the wave's total, summed over the lanes that hold a count (readlane: uniform)
+            for (uint64_t m = __builtin_amdgcn_ballot_w64(rej != 0u); m; m &= m - 1)
+                rejected += (uint32_t)__builtin_amdgcn_readlane((int)rej, __builtin_ctzll(m));
+        }
     };
     // Generation g's population lives in buffer g & 1 of the workspace, so a run split
     // over several launches ([gen_begin, gen_end) each) gives the same result as one.
@@ -186,7 +203,6 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     // A wave evaluates CPL individuals i0, i0+V, ... at a time (one trace loop for all);
     // a slot past NP is evaluated on individual i0's data and discarded.
     for (int i0 = wave; a.gen_begin == 0 && i0 < NP; i0 += kDeWaves * CPL) {
-        float x[CPL][D];
         Candidate kc[CPL];
         bool on[CPL];
 #pragma unroll
@@ -194,21 +210,25 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
             const int i = i0 + c * kDeWaves;
             on[c] = i < NP;
             float u[4];
-            de_uniforms<D>(a, (uint32_t)s, stream, on[c] ? i : i0, 0, u);
+            de_uniforms<D>(a, sample_here(), stream, on[c] ? i : i0, 0, u);
+            float x[D];
+            const int t = thread_index_here();
 #pragma unroll
-            for (int d = 0; d < D; ++d) x[c][d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
-            kc[c] = de_candidate<MODEL>(x[c]);
+            for (int d = 0; d < D; ++d) {
+                x[d] = a.lo[d] + u[d] * (a.hi[d] - a.lo[d]);
+                held[c * D + d][t] = x[d];
+            }
+            kc[c] = de_candidate<MODEL>(x);
         }
         float S[CPL], stk[CPL];
-        unsigned rej = 0;
-        eval(kc, on, S, stk, &rej);
-        if (active) rejected += rej;
+        eval(kc, on, S, stk);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             if (!on[c]) continue;
             const int i = i0 + c * kDeWaves;
+            const int t = thread_index_here();
 #pragma unroll
-            for (int d = 0; d < D; ++d) *rec(0, i, d) = x[c][d];
+            for (int d = 0; d < D; ++d) *rec(0, i, d) = held[c * D + d][t];
             *rec(0, i, D) = S[c];
             *rec(0, i, D + 1) = stk[c];
         }
@@ -218,7 +238,6 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     for (int gen = a.gen_begin > 1 ? a.gen_begin : 1; gen < a.gen_end; ++gen) {
         const int cur = (gen - 1) & 1;
         for (int i0 = wave; i0 < NP; i0 += kDeWaves * CPL) {
-            float trial[CPL][D];
             Candidate kc[CPL];
             bool on[CPL];
 #pragma unroll
@@ -228,41 +247,51 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
                 const int i = on[c] ? i1 : i0;
                 constexpr int NQ = 4 * D + 1;
                 float u[NQ];
-                de_uniforms<NQ>(a, (uint32_t)s, stream, i, gen, u);
+                de_uniforms<NQ>(a, sample_here(), stream, i, gen, u);
+                int np1i = NP - 1;  // donors are drawn from NP - 1 (converted where used)
+                __asm__ volatile("" : "+s"(np1i));
+                const float np1 = (float)np1i;
                 // mutatePopulations (common/differential_evolution.cu:43-89)
                 float mutant[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const unsigned r1 = (unsigned)(u[3 * d] * (float)(NP - 1));
-                    const unsigned r2 = (unsigned)(u[3 * d + 1] * (float)(NP - 1));
-                    const unsigned r3 = (unsigned)(u[3 * d + 2] * (float)(NP - 1));
+                    const unsigned r1 = (unsigned)(u[3 * d] * np1);
+                    const unsigned r2 = (unsigned)(u[3 * d + 1] * np1);
+                    const unsigned r3 = (unsigned)(u[3 * d + 2] * np1);
                     float nv = *rec(cur, r1, d) + kDeF * (*rec(cur, r2, d) - *rec(cur, r3, d));
-                    nv = fminf(nv, a.hi[d]);
-                    nv = fmaxf(nv, a.lo[d]);
+                    // the bounds as opaque scalars: canonicalized at each use (one operation)
+                    // rather than once into vector registers held through the evaluation
+                    int hb = __builtin_amdgcn_readfirstlane(__float_as_int(a.hi[d]));
+                    int lb = __builtin_amdgcn_readfirstlane(__float_as_int(a.lo[d]));
+                    __asm__ volatile("" : "+s"(hb), "+s"(lb));
+                    nv = fminf(nv, __int_as_float(hb));
+                    nv = fmaxf(nv, __int_as_float(lb));
                     mutant[d] = nv;
                 }
                 // crossoverPopulations (common/differential_evolution.cu:91-122)
                 const unsigned l = (unsigned)(u[3 * D] * (float)(D - 1));
+                float trial[D];
+                const int t = thread_index_here();
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const float r = u[3 * D + 1 + d];
-                    trial[c][d] = (r > kDeCR && l != (unsigned)d) ? *rec(cur, i, d) : mutant[d];
+                    trial[d] = (r > kDeCR && l != (unsigned)d) ? *rec(cur, i, d) : mutant[d];
+                    held[c * D + d][t] = trial[d];
                 }
-                kc[c] = de_candidate<MODEL>(trial[c]);
+                kc[c] = de_candidate<MODEL>(trial);
             }
             float S[CPL], stk[CPL];
-            unsigned rej = 0;
-            eval(kc, on, S, stk, &rej);
-            if (active) rejected += rej;
+            eval(kc, on, S, stk);
             // nextGeneration (common/differential_evolution.cu:124-155): strict '>'
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 if (!on[c]) continue;
                 const int i = i0 + c * kDeWaves;
                 const bool take = S[c] > *rec(cur, i, D);
+                const int t = thread_index_here();
 #pragma unroll
                 for (int d = 0; d < D; ++d)
-                    *rec(cur ^ 1, i, d) = take ? trial[c][d] : *rec(cur, i, d);
+                    *rec(cur ^ 1, i, d) = take ? held[c * D + d][t] : *rec(cur, i, d);
                 *rec(cur ^ 1, i, D) = take ? S[c] : *rec(cur, i, D);
                 *rec(cur ^ 1, i, D + 1) = take ? stk[c] : *rec(cur, i, D + 1);
             }
@@ -271,8 +300,8 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
     }
 
     if (MODEL == HSS_OCT && a.not_used) {
-        unsigned long long tot = wave_sum_u64(rejected);
-        if (lane == 0 && tot) atomicAdd(a.not_used + g, tot);
+        if ((thread_index_here() & (kWave - 1)) == 0 && rejected)
+            atomicAdd(a.not_used + g, (unsigned long long)rejected);
     }
     if (a.gen_end < a.G || wave != 0 || !active) return;
     const int cur = (a.G - 1) & 1;

@@ -58,25 +58,6 @@ __device__ __forceinline__ int span_index(float t, float dt, float inv) {
     return (int)ti;
 }
 
-// A wave-uniform float held in a scalar register: readfirstlane, then an empty asm
-// statement with an "s" operand (readfirstlane alone is folded away when the compiler sees
-// that the value is uniform, and the value goes back to a vector register).
-__device__ __forceinline__ float uniform_f(float x) {
-    int b = __builtin_amdgcn_readfirstlane(__float_as_int(x));
-    __asm__("" : "+s"(b));
-    return __int_as_float(b);
-}
-
-// threadIdx.x through an empty asm statement: what is derived from it at a point of use
-// (the lane's LDS slots, its cube row, its output sample) is recomputed there from the one
-// register that holds the thread index, instead of being kept in registers of its own (or
-// spilled) across the trace loops.
-__device__ __forceinline__ int thread_index_here() {
-    int t = (int)threadIdx.x;
-    __asm__ volatile("" : "+v"(t));
-    return t;
-}
-
 // The running best of each thread, in LDS.
 struct RunBest {
     float best[kGridThreads];
