@@ -36,11 +36,14 @@ namespace hss {
 //    memory, and more resident waves pay for the spills of the per-generation code:
 //    OCT 5/6/7/8 waves measured 2435/2334/2356/2479 ms, ZO-CRS 6/7/8 559/557/536 ms.
 //  * de_waves: waves per workgroup, so that whole workgroups fill a CU's wave slots at
-//    that occupancy (OCT 4 x 6 workgroups = 24 per CU; ZO-CRS and CMP 8 x 4 = 32).
+//    that occupancy (OCT 8 x 3 workgroups = 24 per CU; ZO-CRS and CMP 8 x 4 = 32).  For OCT
+//    8 waves were 1.4% faster than 4 x 6 workgroups: a workgroup's generations run half as
+//    long, so the launch's tail (the last workgroups on each XCD, about a quarter of a
+//    second each at 4 waves) is shorter.
 // Kahan windows (twice the accumulators), windows above 11 samples and the runtime-w
 // kernel keep one individual per lane and the compiler's own register budget.
 template <int MODEL>
-constexpr int de_waves() { return MODEL == HSS_OCT ? 4 : 8; }
+constexpr int de_waves() { return 8; }
 template <int MODEL, int W, int VARIANT>
 constexpr bool de_tuned() { return VARIANT != HSS_KAHAN && W != 0 && W <= 11; }
 template <int MODEL, int W, int VARIANT>
