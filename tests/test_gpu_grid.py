@@ -330,6 +330,35 @@ def test_misaligned_16bit_samples_are_rejected():
         capi.grid_semblance(dl, mp, CMP, prm, params.size)
 
 
+def test_cube_larger_than_32_bit_offsets_is_refused():
+    """The per-candidate cubes are addressed per midpoint with 32-bit offsets: a request for
+    them with ns * P >= 2^32 is refused with HSS_ERR_INVALID_SIZE before any launch (the
+    cube pointers are never touched, so small buffers stand in for them), and the same call
+    without cubes runs."""
+    import torch
+    line = synth.make_line(1, 4, 500.0, 8192, 2000, seed=5)
+    tau = synth.tau_index(0.011, line.dt_us)
+    P = (1 << 19) + 1
+    assert line.ns * P > 0xFFFFFFFF
+    lo, hi, G = [1400.0], [5000.0], [P]
+    params = O.build_parameter_array(O.CMP, lo, grid_steps(lo, hi, G), G)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau)
+    b, c, m0 = line.cmp_ranges()
+    mp = capi.DeviceMidpoints(b, c, m0)
+    prm = torch.from_numpy(np.ascontiguousarray(params)).cuda()
+    result = torch.empty((mp.n, 3, line.ns), dtype=torch.float32, device="cuda")
+    stand_in = torch.empty(16, dtype=torch.float32, device="cuda")
+    rc = capi.lib().hss_grid_semblance(
+        capi.ctypes.byref(dl.struct), capi.ctypes.byref(mp.struct), CMP, capi._ptr(prm), P,
+        FAST, capi._ptr(result), capi._ptr(stand_in), None, None, None)
+    assert rc == -3
+    rc = capi.lib().hss_grid_semblance(
+        capi.ctypes.byref(dl.struct), capi.ctypes.byref(mp.struct), CMP, capi._ptr(prm), P,
+        FAST, capi._ptr(result), None, None, None, None)
+    torch.cuda.synchronize()
+    assert rc == 0
+
+
 def _nan_aware_equal(a, b):
     """Same NaN positions, and the other values bit for bit (a NaN's payload is not
     compared: 0/0 gives different NaN bits on the CPU and the GPU)."""
