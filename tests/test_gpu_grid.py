@@ -219,6 +219,47 @@ def test_staged_shape_is_refused_where_it_does_not_apply():
         capi.grid_set_shape(CMP, 11, FAST, F32, 2, 2, 1)
 
 
+@pytest.mark.parametrize("variant", [EXACT, FAST, KAHAN])
+@pytest.mark.parametrize("case", ["partial_tile", "wide_window", "empty_and_muted"])
+def test_staged_shape_edge_cases(case, variant):
+    """The workgroup-staged CMP shapes give the default shape's results bit for bit (NaN
+    positions where the semblance is 0/0) on the edges of their staging: a line shorter
+    than one 64-sample t0 tile; a window above 15 samples (w = 21, the runtime-w kernel,
+    whose far traces overflow the 128-sample stage and take the direct reads); a batch
+    with an empty midpoint and a muted zone."""
+    if case == "partial_tile":
+        line = synth.make_line(3, (20, 40), 2500.0, 40, 4000, seed=31, offset_min=50.0)
+        tau = synth.tau_index(0.012, line.dt_us)
+        b, c, m0 = line.cmp_ranges()
+    elif case == "wide_window":
+        line = synth.make_line(3, (20, 40), 2500.0, 301, 4000, seed=32, offset_min=50.0)
+        tau = 10
+        b, c, m0 = line.cmp_ranges()
+    else:
+        line = synth.make_line(2, 24, 2000.0, 301, 4000, seed=5, offset_min=50.0)
+        line.samples[:, :120] = 0.0
+        tau = synth.tau_index(0.02, line.dt_us)
+        b, c, m0 = line.cmp_ranges()
+        b, c = np.append(b, 0).astype(np.int32), np.append(c, 0).astype(np.int32)
+        m0 = np.append(m0, 1000.0).astype(np.float32)
+    lo, hi, G = [1500.0], [4500.0], [37]
+    params = O.build_parameter_array(O.CMP, lo, grid_steps(lo, hi, G), G)
+    w = 2 * tau + 1
+    out = {}
+    try:
+        for shape in [(1, 1, 0), (1, 1, 1), (2, 1, 1)]:
+            capi.grid_set_shape(CMP, w, variant, F32, *shape)
+            out[shape] = run_gpu(line, b, c, m0, CMP, params, tau, variant)
+    finally:
+        capi.grid_set_shape(CMP, w, variant, F32, 0, 0, 0)
+    if case == "empty_and_muted":
+        assert np.isnan(out[(1, 1, 0)][1]).any()
+        assert np.all(out[(1, 1, 0)][1][-1] == 0)
+    for key, r in out.items():
+        for i in range(3):
+            assert _nan_aware_equal(out[(1, 1, 0)][i], r[i]), (case, key, i)
+
+
 def test_line_batch_equals_single_midpoints():
     """A batch of midpoints in one launch gives, per midpoint, exactly the values of a
     one-midpoint launch (the reference's per-CDP call)."""
