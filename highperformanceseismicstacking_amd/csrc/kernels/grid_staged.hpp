@@ -15,9 +15,9 @@
 // every operation of the index chain are nondecreasing in t0 and in c): the smallest
 // window start of the round is at the tile's first t0 and the round's smallest c, the
 // largest at its last t0 and largest c.  A block with a trace whose span exceeds the stage
-// takes direct reads instead.  Each lane adds its traces in the same order with the same operations on
-// the same values as the grid kernel, so the results are the same bits (HSS_KAHAN's
-// chunk boundaries are the same 16-trace blocks).
+// takes direct reads instead.  Each lane adds its traces in the same order with the same
+// operations on the same values as the grid kernel, so the results are the same bits
+// (HSS_KAHAN's chunk boundaries are the same 16-trace blocks).
 #pragma once
 
 #include "grid_impl.hpp"
