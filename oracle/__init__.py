@@ -50,7 +50,7 @@ def lib():
         L.orc_stretch_free_f32.argtypes = [i, vp, vp, vp, i, vp, vp, vp, i]
         L.orc_stretch_free_f64.argtypes = [i, vp, vp, vp, i, vp, i]
         L.orc_de_f32.argtypes = [i, vp, vp, vp, i, i, ctypes.c_uint64, ctypes.c_uint32,
-                                 vp, vp, vp, i]
+                                 vp, vp, vp, vp, i]
         L.orc_de_start_individual.argtypes = [i, vp, vp, vp, vp]
         L.orc_de_trial.argtypes = [i, i, vp, i, vp, vp, vp, vp, vp]
         L.orc_select_traces.argtypes = [i, vp, vp, i, fl, fl, fl, i, fl, vp, i, i, vp]
@@ -164,16 +164,23 @@ def stretch_free_f64(model, g, vparams, nvals, nthreads=8):
     return res
 
 
-def de_f32(model, g, lower, upper, NP, G, seed, stream_id=0, nthreads=8, population=False):
+def de_f32(model, g, lower, upper, NP, G, seed, stream_id=0, nthreads=8, population=False,
+           not_used=False):
+    """result [(2+np)][ns]; with population=True also the final population x, fx; with
+    not_used=True also the traces the model rejected over the whole run (last item)."""
     n = NPARAMS[model]
     lo = np.ascontiguousarray(lower[:n], dtype=np.float32)
     hi = np.ascontiguousarray(upper[:n], dtype=np.float32)
     res = np.empty((2 + n, g.ns), np.float32)
     x = np.empty((g.ns, NP, n), np.float32) if population else None
     fx = np.empty((g.ns, NP, 2), np.float32) if population else None
+    nu = np.zeros(1, np.uint64) if not_used else None
     lib().orc_de_f32(model, g.ref, _p(lo), _p(hi), NP, G, seed, stream_id, _p(res), _p(x), _p(fx),
-                     nthreads)
-    return (res, x, fx) if population else res
+                     _p(nu), nthreads)
+    out = (res, x, fx) if population else (res,)
+    if not_used:
+        out = out + (int(nu[0]),)
+    return out if len(out) > 1 else out[0]
 
 
 def de_start_individual(lower, upper, u):

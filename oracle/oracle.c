@@ -546,7 +546,7 @@ static void orc_de_uniforms(uint64_t seed, uint32_t stream_id, int s, int i, int
 }
 
 static void orc_de_eval(int model, const orc_gather *g, int s, const float *xv,
-                        float *S, float *stk)
+                        float *S, float *stk, uint64_t *not_used)
 {
     float prm[3];
     float v = xv[0];
@@ -557,7 +557,9 @@ static void orc_de_eval(int model, const orc_gather *g, int s, const float *xv,
     } else if (model == ORC_OCT) {
         prm[1] = xv[1];
     }
-    orc_eval_f32(model, g, s, 0, prm, S, stk, NULL, NULL);
+    unsigned nu = 0;
+    orc_eval_f32(model, g, s, 0, prm, S, stk, NULL, &nu);
+    *not_used += nu;
 }
 
 /* One start individual from D uniforms: x_d = lower_d + U_d * (upper_d - lower_d)
@@ -597,13 +599,14 @@ void orc_de_trial(int D, int NP, const float *x, int i, const float *lower, cons
 
 void orc_de_f32(int model, const orc_gather *g, const float *lower, const float *upper,
                 int NP, int G, uint64_t seed, uint32_t stream_id, float *result,
-                float *x_out, float *fx_out, int nthreads)
+                float *x_out, float *fx_out, uint64_t *not_used_total, int nthreads)
 {
     const int D = orc_model_np(model);
     const int ns = g->ns;
     if (nthreads <= 0) nthreads = 1;
     (void)nthreads;
-#pragma omp parallel num_threads(nthreads)
+    uint64_t nu_all = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : nu_all)
     {
         float *x = (float *)malloc(sizeof(float) * NP * D);
         float *vv = (float *)malloc(sizeof(float) * NP * D);
@@ -618,7 +621,7 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
                 orc_de_start_individual(D, lower, upper, un, x + i * D);
             }
             for (int i = 0; i < NP; i++) {
-                orc_de_eval(model, g, s, x + i * D, &fx[2 * i], &fx[2 * i + 1]);
+                orc_de_eval(model, g, s, x + i * D, &fx[2 * i], &fx[2 * i + 1], &nu_all);
             }
             for (int gen = 1; gen < G; gen++) {
                 /* synchronous generation: every trial is formed from the population of
@@ -628,7 +631,7 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
                     orc_de_trial(D, NP, x, i, lower, upper, un, vv + i * D, uu + i * D);
                 }
                 for (int i = 0; i < NP; i++) {
-                    orc_de_eval(model, g, s, uu + i * D, &fu[2 * i], &fu[2 * i + 1]);
+                    orc_de_eval(model, g, s, uu + i * D, &fu[2 * i], &fu[2 * i + 1], &nu_all);
                 }
                 for (int i = 0; i < NP; i++) {
                     if (fu[2 * i] > fx[2 * i]) {
@@ -655,6 +658,7 @@ void orc_de_f32(int model, const orc_gather *g, const float *lower, const float 
         }
         free(x); free(vv); free(uu); free(fx); free(fu);
     }
+    if (not_used_total) *not_used_total = nu_all;
 }
 
 /* ------------------------------------------------------------------------- */

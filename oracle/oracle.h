@@ -108,10 +108,12 @@ void orc_stretch_free_f64(int model, const orc_gather *g, const float *vparams,
 /* ---- differential evolution over one midpoint with a Philox stream
  * (common/differential_evolution.cu:19-155, cmp|zocrs|oct/differential_evolution.cu).
  * lower/upper [np] in v space.  result [(2+np)][ns].
- * x_out [ns][NP][np], fx_out [ns][NP][2] (nullable): final population. */
+ * x_out [ns][NP][np], fx_out [ns][NP][2] (nullable): final population.
+ * not_used_total (nullable): the traces the model rejected, summed over every evaluation
+ * of the run (the start population and each generation's trials) and every sample. */
 void orc_de_f32(int model, const orc_gather *g, const float *lower, const float *upper,
                 int NP, int G, uint64_t seed, uint32_t stream_id, float *result,
-                float *x_out, float *fx_out, int nthreads);
+                float *x_out, float *fx_out, uint64_t *not_used_total, int nthreads);
 
 /* The DE operators on explicit uniforms (the pieces orc_de_f32 is made of):
  * orc_de_start_individual: x_d = lower_d + u[d] * (upper_d - lower_d);

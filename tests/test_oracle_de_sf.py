@@ -330,3 +330,31 @@ def test_stretch_free_picks_the_first_strict_maximum_over_shifts(model):
         if bq >= 0:
             assert res[2, s] == nvals[bq] and _bits(res[1, s]) == _bits(stacks[bq, s])
             assert bq != 3    # the repeated shift never wins over its first copy
+
+
+def test_de_rejection_total_is_the_sum_over_its_evaluations():
+    """The run's OCT rejection total (what the GPU accumulates into notUsed) against the
+    pinned single evaluation: with G = 1 the final population is the start population, and
+    the total is the single evaluations' counts summed over every sample and individual; a
+    longer run adds its trials' rejections to those of the same start population."""
+    rng = np.random.default_rng(23)
+    ntr, ns = 40, 60
+    samples = rng.normal(0, 1, (ntr, ns)).astype(np.float32)
+    mid = (1000.0 + rng.uniform(-120, 120, ntr)).astype(np.float32)
+    hoff = rng.uniform(-1500, 1500, ntr).astype(np.float32)
+    g = O.Gather(samples, mid, hoff, 0.004, 3, m0=1000.0, h0=150.0, apm=40.0)
+    lo, hi = [1500.0, -5e-4], [4500.0, 5e-4]
+    NP = 6
+    r, x, fx, nu1 = O.de_f32(O.OCT, g, lo, hi, NP, 1, 77, stream_id=3, population=True,
+                             not_used=True)
+    want = 0
+    for s in range(ns):
+        for i in range(NP):
+            v, slope = x[s, i]
+            c = f32(4.0) / (f32(v) * f32(v))
+            S, T, used, nu = O.eval_f32(O.OCT, g, s, [c, slope])
+            assert S == fx[s, i, 0] or (np.isnan(S) and np.isnan(fx[s, i, 0]))
+            want += nu
+    assert 0 < nu1 == want < ns * NP * ntr
+    _, nu4 = O.de_f32(O.OCT, g, lo, hi, NP, 4, 77, stream_id=3, not_used=True)
+    assert nu1 < nu4 <= 4 * ns * NP * ntr
