@@ -52,17 +52,46 @@ def test_de_exact_in_offset_order_reproduces_oracle_trajectory(storage, h0):
     NP, G, seed, base = 8, 5, 0xC0FFEE, 11
     dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
     mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    nu = torch.zeros(mp.n, dtype=torch.int64, device="cuda")
     res, px, pfx = capi.de_semblance(dl, mp, OCT, lo, hi, NP, G, seed, stream_base=base,
-                                     variant=EXACT, population=True)
+                                     variant=EXACT, population=True, not_used=nu)
     torch.cuda.synchronize()
     res, px, pfx = res.cpu().numpy(), px.cpu().numpy(), pfx.cpu().numpy()
+    nu = nu.cpu().numpy()
     samples = dl.decoded if storage != F32 else line.samples
     for g in range(len(m0)):
         og = _gather(samples, line, lists[g], tau, float(m0[g]), h0, apm)
-        r, x, fx = O.de_f32(OCT, og, lo, hi, NP, G, seed, stream_id=base + g, population=True)
+        r, x, fx, rnu = O.de_f32(OCT, og, lo, hi, NP, G, seed, stream_id=base + g,
+                                 population=True, not_used=True)
         assert np.array_equal(_bits(px[g]), _bits(x)), g
         assert np.array_equal(_bits(pfx[g]), _bits(fx)), g
         assert np.array_equal(_bits(res[g]), _bits(r)), g
+        assert rnu > 0 and int(nu[g]) == rnu, g
+
+
+@pytest.mark.parametrize("storage", [F32, F16])
+@pytest.mark.parametrize("h0", [0.0, 150.0])
+def test_de_fast_start_population_rejections_equal_oracle(storage, h0):
+    """FAST DE walks each lane's kept traces (the compacted walk) and counts the traces the
+    aperture rejects from the decisions of its first pass.  With G = 1 the individuals are
+    the start population -- the same Philox draws as the oracle's -- and the decisions come
+    from the IEEE traveltime chain in both, so the counts are the oracle's exactly."""
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(h0)
+    tau = synth.tau_index(0.02, 4000)
+    NP, seed, base = 24, 0xBEEF, 5
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    nu = torch.zeros(mp.n, dtype=torch.int64, device="cuda")
+    capi.de_semblance(dl, mp, OCT, lo, hi, NP, 1, seed, stream_base=base, variant=FAST,
+                      not_used=nu)
+    torch.cuda.synchronize()
+    nu = nu.cpu().numpy()
+    samples = dl.decoded if storage != F32 else line.samples
+    for g in range(len(m0)):
+        og = _gather(samples, line, lists[g], tau, float(m0[g]), h0, apm)
+        _, rnu = O.de_f32(OCT, og, lo, hi, NP, 1, seed, stream_id=base + g, not_used=True)
+        assert rnu > 0 and int(nu[g]) == rnu, g
 
 
 @pytest.mark.parametrize("h0", [0.0, 150.0])
