@@ -177,7 +177,9 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
                (thread_index_here() & (kWave - 1));
     };
     // the lane's sample, as the counter of its random stream
-    auto sample_here = [&]() { return (uint32_t)(tile * kWave + (thread_index_here() & (kWave - 1))); };
+    auto sample_here = [&]() {
+        return (uint32_t)(tile * kWave + (thread_index_here() & (kWave - 1)));
+    };
 
     __shared__ float seg_lds[kDeWaves][kSegFloats];
     __shared__ OctTraceRec trace_recs[kDeWaves][kWave];
