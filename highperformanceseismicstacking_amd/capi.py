@@ -139,23 +139,26 @@ def pow2_scale(samples):
 def encode_samples(samples, storage):
     """Returns (stored array as numpy, per-trace scale or None, decoded float32 copy).
 
-    F16/BF16 store samples/scale with a power-of-two per-trace scale; the decoded copy
-    is float(stored) * scale computed in float32 -- the values the kernels read."""
+    F16 stores samples/scale with a power-of-two per-trace scale (FP16's range); the
+    decoded copy is float(stored) * scale computed in float32 -- the values the kernels
+    read.  BF16 has FP32's exponent range, where a power-of-two scale changes no rounded
+    bit, so it stores the samples themselves with no scale (and the kernels sum its windows
+    as FP32 windows)."""
     samples = np.ascontiguousarray(samples, dtype=np.float32)
     if storage == F32:
         return samples, None, samples
-    scale = pow2_scale(samples)
-    norm = (samples / scale[:, None]).astype(np.float32)
     if storage == F16:
+        scale = pow2_scale(samples)
+        norm = (samples / scale[:, None]).astype(np.float32)
         stored = norm.astype(np.float16)
         dec = stored.astype(np.float32) * scale[:, None]
         return stored, scale, dec.astype(np.float32)
     if storage == BF16:
-        u = norm.view(np.uint32)
+        u = samples.view(np.uint32)
         # round to nearest even on the upper 16 bits
         rounded = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
-        dec = (rounded.astype(np.uint32) << 16).view(np.float32) * scale[:, None]
-        return rounded, scale, dec.astype(np.float32)
+        dec = (rounded.astype(np.uint32) << 16).view(np.float32)
+        return rounded, None, dec.astype(np.float32)
     raise ValueError(storage)
 
 

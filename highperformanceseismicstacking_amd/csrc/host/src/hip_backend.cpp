@@ -26,8 +26,10 @@ static hipStream_t stream_of(const std::shared_ptr<DeviceContext> &ctx) {
 }
 
 // ---------------------------------------------------------------------------------
-// Sample storage: F16/BF16 keep samples/scale with a power-of-two scale per trace, so
-// the scale itself adds no rounding; the kernels read float(stored) * scale.
+// Sample storage: F16 keeps samples/scale with a power-of-two scale per trace (FP16's
+// range), so the scale itself adds no rounding; the kernels read float(stored) * scale.
+// BF16 has FP32's exponent range, where such a scale changes no rounded bit: it keeps the
+// samples themselves, with no scale.
 static float pow2_scale(const float *x, unsigned n) {
     float amax = 0.0f;
     for (unsigned i = 0; i < n; i++) amax = std::max(amax, std::fabs(x[i]));
@@ -72,20 +74,22 @@ void HipLine::upload(const Gather &gather, int storage, hipStream_t stream) {
         dSamples.upload(s.data(), n * sizeof(float), stream);
         dScale.resize(0);
     } else {
+        const bool scaled = storage == HSS_F16;
         std::vector<uint16_t> s(n);
-        std::vector<float> scale(ntr);
+        std::vector<float> scale(scaled ? ntr : 0);
         for (unsigned i = 0; i < ntr; i++) {
             const auto &x = gather.getTraceAtIndex(i).getSamples();
-            scale[i] = pow2_scale(x.data(), ns);
+            const float sc = scaled ? pow2_scale(x.data(), ns) : 1.0f;
+            if (scaled) scale[i] = sc;
             for (unsigned k = 0; k < ns; k++) {
-                const float v = x[k] / scale[i];
-                s[static_cast<size_t>(i) * ns + k] = storage == HSS_F16 ? to_f16(v) : to_bf16(v);
+                const float v = scaled ? x[k] / sc : x[k];
+                s[static_cast<size_t>(i) * ns + k] = scaled ? to_f16(v) : to_bf16(v);
             }
         }
         dSamples.resize(n * sizeof(uint16_t));
         dSamples.upload(s.data(), n * sizeof(uint16_t), stream);
-        dScale.resize(ntr * sizeof(float));
-        dScale.upload(scale.data(), ntr * sizeof(float), stream);
+        dScale.resize(scale.size() * sizeof(float));
+        if (scaled) dScale.upload(scale.data(), ntr * sizeof(float), stream);
     }
     dMid.resize(ntr * sizeof(float));
     dMid.upload(mid.data(), ntr * sizeof(float), stream);
@@ -95,7 +99,7 @@ void HipLine::upload(const Gather &gather, int storage, hipStream_t stream) {
     dHoff2.upload(hoff2.data(), ntr * sizeof(float), stream);
     view = hss_traces{};
     view.samples = dSamples.get();
-    view.scale = storage == HSS_F32 ? nullptr : dScale.as<float>();
+    view.scale = storage == HSS_F16 ? dScale.as<float>() : nullptr;
     view.mid = dMid.as<float>();
     view.hoff = dHoff.as<float>();
     view.hoff2 = dHoff2.as<float>();

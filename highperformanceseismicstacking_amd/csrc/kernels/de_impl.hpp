@@ -193,6 +193,11 @@ __attribute__((amdgpu_waves_per_eu(de_min_occupancy<MODEL, W, VARIANT>(), 8))) v
         if constexpr (de_compact<MODEL, VARIANT>())
             evaluate_oct_compact<W, VARIANT, STORAGE, false, CPL>(
                 e, t0, kc, on, 0.0f, active, trace_recs[wave], S, stk, &rej);
+        else if (STORAGE == HSS_BF16 && e.scale == nullptr)
+            // a BF16 line stored without scales: its windows summed as FP32 windows (one
+            // form per code path, chosen outside the trace loop)
+            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep, false>(
+                e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);
         else
             evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, kDeSegments, kDeOctKeep>(
                 e, t0, t0sq, kc, on, 0.0f, active, seg_lds[wave], S, stk, &rej);

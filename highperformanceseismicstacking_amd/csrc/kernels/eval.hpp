@@ -123,8 +123,10 @@ __device__ __forceinline__ void load_segment(const EvalCtx &e, size_t g0, int li
 // OCT_KEEP (OCT only): keep the offset-dependent half of the traveltime across
 // consecutive traces with the same half offset (for lists grouped by offset; in the
 // reference's order it only adds register pressure).
+// FACTORED16 (16-bit per-lane windows): Window::add16's form; false only for a line
+// stored without scales (the caller checks e.scale == nullptr).
 template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, int CPL,
-          bool SEG = (STORAGE != HSS_F32), bool OCT_KEEP = false>
+          bool SEG = (STORAGE != HSS_F32), bool OCT_KEEP = false, bool FACTORED16 = true>
 __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0sq,
                                            const Candidate (&k)[CPL], const bool (&on)[CPL],
                                            float shift, bool active, float *seg_generic,
@@ -350,7 +352,9 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
                 const uint32_t *roww = (const uint32_t *)((const uint16_t *)e.samples + (row - rpar));
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    if (use[c]) acc[c].template add16<STORAGE>(roww, rpar, (unsigned)kk[c], f[c], sc);
+                    if (use[c])
+                        acc[c].template add16<STORAGE, FACTORED16>(roww, rpar, (unsigned)kk[c],
+                                                                   f[c], sc);
                 }
             } else {
 #pragma unroll
@@ -370,7 +374,8 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
 }
 
 // One candidate per lane.
-template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32)>
+template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32),
+          bool FACTORED16 = true>
 __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
                                          const Candidate &k, float shift, bool active,
                                          float *seg_generic, float *S, float *stack,
@@ -378,8 +383,8 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
     const Candidate kk[1] = {k};
     const bool on[1] = {true};
     float s1[1], t1[1];
-    evaluate_n<MODEL, W, VARIANT, STORAGE, SHIFT, 1, SEG>(e, t0, t0sq, kk, on, shift, active,
-                                                          seg_generic, s1, t1, rejected);
+    evaluate_n<MODEL, W, VARIANT, STORAGE, SHIFT, 1, SEG, false, FACTORED16>(
+        e, t0, t0sq, kk, on, shift, active, seg_generic, s1, t1, rejected);
     *S = s1[0];
     *stack = t1[0];
 }

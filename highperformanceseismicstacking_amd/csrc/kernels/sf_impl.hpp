@@ -79,8 +79,14 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
         const float t0n = (float)sc * e.dt;
         float S = 0.0f, stk = 0.0f;
         unsigned rej = 0;
-        evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments>(
-            e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
+        // a BF16 line stored without scales sums its windows as FP32 windows (one form per
+        // code path, chosen outside the trace loop)
+        if (STORAGE == HSS_BF16 && e.scale == nullptr)
+            evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments, false>(
+                e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
+        else
+            evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments>(
+                e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
         if (!valid) {
             S = 0.0f;
             stk = 0.0f;

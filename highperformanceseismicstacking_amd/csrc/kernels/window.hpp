@@ -198,12 +198,17 @@ struct Window {
         used++;
     }
 
-    // 16-bit samples of one window read per lane (read16), then summed.
-    template <int STORAGE>
+    // 16-bit samples of one window read per lane (read16), then summed.  FACTORED: the
+    // non-exact variants factor the trace's scale `sc` out of the window (a line without
+    // scales passes sc = 1).  !FACTORED, for a line stored without scales (BF16: FP32's
+    // exponent range needs none): the FP32 steps, whose sums need an add per value where
+    // the factored scale needs an FMA.  The choice is a template parameter so that a
+    // kernel holds one form in its trace loop (both would add registers).
+    template <int STORAGE, bool FACTORED = true>
     __device__ __forceinline__ void add16(const uint32_t *roww, unsigned rpar, unsigned k,
                                           float f, float sc) {
         static_assert(W > 0, "add16 needs a compiled window size");
-        if constexpr (VARIANT != HSS_EXACT && STORAGE == HSS_F16) {
+        if constexpr (VARIANT != HSS_EXACT && STORAGE == HSS_F16 && FACTORED) {
             // as below, with y1 - y0 formed as fma(y1, one, -y0) = RN(y1 - y0): the same
             // bits, but an FMA reads both halves straight from the words (v_fma_mix_f32),
             // where a subtraction needs both converted first.  `one` is 1.0 hidden from the
@@ -228,6 +233,12 @@ struct Window {
         float y[W + 1];
         read16<STORAGE, W>(roww, rpar, k, y);
         if constexpr (VARIANT != HSS_EXACT) {
+            if constexpr (!FACTORED) {
+#pragma unroll
+                for (int j = 0; j < W; ++j) step(j, y[j], y[j + 1], f);
+                used++;
+                return;
+            }
             // the trace's scale factored out of the window: u = sc * (y0 + frac*(y1 - y0))
             // on the stored values, A_j += sc*u and den += sc^2 * sum_j u^2
             float dn = 0.0f;
@@ -239,8 +250,10 @@ struct Window {
             }
             den = __builtin_fmaf(sc * sc, dn, den);
         } else {
+            if constexpr (FACTORED) {
 #pragma unroll
-            for (int j = 0; j <= W; ++j) y[j] = y[j] * sc;
+                for (int j = 0; j <= W; ++j) y[j] = y[j] * sc;
+            }
 #pragma unroll
             for (int j = 0; j < W; ++j) step(j, y[j], y[j + 1], f);
         }
