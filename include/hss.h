@@ -21,7 +21,9 @@
  *     and waits for it (outside stream capture: hss_index_ops_state); later calls are
  *     fully asynchronous;
  *   - floating point in/out is float32; trace samples may be stored as float32,
- *     float16 or bfloat16 (hss_traces.storage) with a per-trace float32 scale.
+ *     float16 or bfloat16 (hss_traces.storage) with an optional per-trace float32 scale
+ *     (float16 needs one for range; bfloat16, with float32's exponent range, is best
+ *     stored without: its windows are then summed as float32 windows).
  *
  * Model ids follow traveltime_t (common/include/capability.h:3-7).
  */
