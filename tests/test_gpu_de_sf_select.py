@@ -276,3 +276,73 @@ def test_selection_oct_three_candidate_sources():
                               pairs=pr, npairs=NP, per_sample=True)
         assert np.array_equal(mask[g], ref), g
         assert 0 < ref.sum() < line.ntr
+
+
+def _attach_pow2_scales(dl, samples):
+    """Re-store a BF16 DeviceLine as samples over a power-of-two scale per trace with the
+    scales attached (the form the C ABI still takes for BF16); returns the decoded values."""
+    import torch
+    scale = capi.pow2_scale(samples)
+    norm = (samples / scale[:, None]).astype(np.float32)
+    u = norm.view(np.uint32)
+    stored = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    dec = ((stored.astype(np.uint32) << 16).view(np.float32) * scale[:, None]).astype(np.float32)
+    dl.samples_scaled = torch.from_numpy(stored.view(np.int16)).cuda()
+    dl.scale_t = torch.from_numpy(scale).cuda()
+    dl.struct.samples = dl.samples_scaled.data_ptr()
+    dl.struct.scale = dl.scale_t.data_ptr()
+    return dec
+
+
+@pytest.mark.parametrize("variant", [EXACT, FAST, KAHAN])
+def test_bf16_with_scales_equals_bf16_without(variant):
+    """BF16 lines are stored without scales, and the differential-evolution and
+    stretch-free kernels then sum their windows with the FP32 steps; per-trace scales (which
+    the ABI still takes) select the factored form.  Power-of-two scales change no decoded
+    bit, so: the grid (which scales at decode) gives the same bits in every variant, DE and
+    stretch-free give the same bits in the exact variant, and stretch-free's other variants
+    agree within 1e-5."""
+    import torch
+    line, b, c, m0, lo, hi, h0, apm, tau = de_case(ZOCRS)
+    plain = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=BF16)
+    scaled = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=BF16)
+    dec = _attach_pow2_scales(scaled, np.asarray(line.samples, dtype=np.float32))
+    assert plain.scale is None and np.array_equal(_bits(dec), _bits(plain.decoded))
+    mp = capi.DeviceMidpoints(b, c, m0, h0=h0, apm=apm)
+
+    # grid search (CMP over the same traces)
+    cb, cc, cm0 = line.cmp_ranges()
+    cmp_mp = capi.DeviceMidpoints(cb[:3], cc[:3], cm0[:3])
+    step = [f32((hi[0] - lo[0]) / 40.0)]
+    params = capi.build_parameter_array(CMP, lo[:1], step, [40])
+    g1 = capi.grid_semblance(plain, cmp_mp, CMP, params, 40, variant=variant, full=True)
+    g2 = capi.grid_semblance(scaled, cmp_mp, CMP, params, 40, variant=variant, full=True)
+    torch.cuda.synchronize()
+    for i in range(3):
+        assert np.array_equal(_bits(g1[i].cpu().numpy()), _bits(g2[i].cpu().numpy())), i
+
+    # differential evolution (ZO-CRS, the C3 path)
+    if variant == EXACT:
+        d1, _, _ = capi.de_semblance(plain, mp, ZOCRS, lo, hi, 8, 5, 99, variant=EXACT)
+        d2, _, _ = capi.de_semblance(scaled, mp, ZOCRS, lo, hi, 8, 5, 99, variant=EXACT)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(d1.cpu().numpy()), _bits(d2.cpu().numpy()))
+
+    # stretch-free (ZO-CRS) at parameters spread over the bounds
+    rng = np.random.default_rng(8)
+    vp = np.empty((len(b), 3, line.ns), np.float32)
+    for d in range(3):
+        vp[:, d] = rng.uniform(lo[d], hi[d], (len(b), line.ns))
+    nvals = np.arange(-3, 4, dtype=np.float32)
+    vpt = torch.from_numpy(vp).cuda()
+    s1 = capi.stretch_free(plain, mp, ZOCRS, vpt, nvals, variant=variant, full=True)
+    s2 = capi.stretch_free(scaled, mp, ZOCRS, vpt, nvals, variant=variant, full=True)
+    torch.cuda.synchronize()
+    S1, T1, S2, T2 = (x.cpu().numpy() for x in (s1[1], s1[2], s2[1], s2[2]))
+    if variant == EXACT:
+        assert np.array_equal(_bits(S1), _bits(S2)) and np.array_equal(_bits(T1), _bits(T2))
+    else:
+        okS, okT = compare_close(S2, T2, S1, T1)
+        okS |= np.isnan(S1) & np.isnan(S2)  # 0/0 windows on both sides
+        okT |= np.isnan(T1) & np.isnan(T2)
+        assert okS.all() and okT.all()
