@@ -14,7 +14,8 @@ the FP16-storage step; the EXACT (reference-bits) grid launch against the plain 
 FP32 EXACT kernel (the reference's kernel structure); the precision path (FP16 storage,
 compensated window sums, then 10 Gauss-Newton steps of the adjoint polish); the C2 line
 (500 midpoints, 1024 velocities); the ZO-CRS DE line time (C3) and the OCT DE line time
-(C4); and a device-to-device copy as the measured HBM reference.
+(C4), each with a grid search over the same midpoints and traces (a candidate grid across
+the DE bounds); and a device-to-device copy as the measured HBM reference.
 
 Roofline of the dominant kernel (the grid launch; HIP events on its stream, inside the
 timed region): the kernel is bound by its FP32 vector instruction stream, so `achieved`
@@ -445,14 +446,17 @@ def other_workloads(args):
         de_args = argparse.Namespace(**vars(args))
         de_args.workload, de_args.storage, de_args.variant = wl, storage, "fast"
         de_args.steps, de_args.warmup, de_args.calibrate_copy_mb = 3, 1, 0
+        de_args.grid_leg = True
         de = run_de(de_args)
         key = "zocrs_de" if wl == "c3" else "oct_de"
-        progress(f"{wl} line {de['config']['line_time_s']:.3f} s")
+        progress(f"{wl} line {de['config']['line_time_s']:.3f} s, grid "
+                 f"{de['config']['grid_search']['launch_ms']:.1f} ms")
         out[key] = {"workload": de["config"]["workload"],
                     "line_time_s": de["config"]["line_time_s"],
                     "de_launch_ms": de["config"]["de_launch_ms"],
                     "polish_launch_ms": de["config"]["polish_launch_ms"],
-                    "evals_per_s": de["value"]}
+                    "evals_per_s": de["value"],
+                    "grid_search": de["config"]["grid_search"]}
         torch.cuda.empty_cache()
     return out
 
@@ -667,6 +671,29 @@ def run_de(args):
         "roofline": roofline(flops, alg_bytes, de_ms, prof, None),
         "cpu_baseline": None,
     }
+    if getattr(args, "grid_leg", False):
+        # grid search over the same midpoints and selected traces: a candidate grid that
+        # spans the DE bounds (the variant and storage of the DE run)
+        grid_counts = (16, 8, 8) if model == ZOCRS else (64, 16)
+        grid_steps = [float(f32(f32(upper[k] - lower[k]) / f32(grid_counts[k])))
+                      for k in range(np_)]
+        P = int(np.prod(grid_counts))
+        gparams = capi.build_parameter_array(model, lower, grid_steps, list(grid_counts))
+        gres = torch.empty((mp.n, 2 + np_, line.ns), dtype=torch.float32, device="cuda")
+
+        def gr():
+            capi.grid_semblance(dl, mp, model, gparams, P, variant=variant, result=gres,
+                                stream=stream)
+
+        gr()  # the first launch of at least 4096 workgroups tunes the launch shape
+        torch.cuda.synchronize()
+        grid_ms = time_region(gr, max(2, args.steps), 1, stream)
+        out["config"]["grid_search"] = {
+            "candidates": " x ".join(str(x) for x in grid_counts) + " over the DE bounds",
+            "P": P,
+            "launch_ms": grid_ms,
+            "evals_per_s": int(mp.n) * line.ns * P / (grid_ms * 1e-3),
+        }
     calibration_copy(args.calibrate_copy_mb)
     return out
 

@@ -69,7 +69,10 @@ int tuned_grid_shape(int model, int w, int variant, int storage, int n_slots, in
             return HSS_OK;
         }
     }
-    *shape_out = GridShape{1, 1, 0};
+    // untuned launches (small, under capture, tuning off): two candidates per lane, the
+    // choice of every tuned line but the compensated variant's, which keeps one (its
+    // two-candidate forms spill)
+    *shape_out = GridShape{variant == HSS_KAHAN ? 1 : 2, 1, 0};
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return HSS_OK;
