@@ -69,10 +69,12 @@ int tuned_grid_shape(int model, int w, int variant, int storage, int n_slots, in
             return HSS_OK;
         }
     }
-    // untuned launches (small, under capture, tuning off): two candidates per lane, the
-    // choice of every tuned line but the compensated variant's, which keeps one (its
-    // two-candidate forms spill)
-    *shape_out = GridShape{variant == HSS_KAHAN ? 1 : 2, 1, 0};
+    // untuned launches (small, under capture, tuning off): the choice of every tuned line
+    // -- two candidates per lane, for CMP with the workgroup-staged traces -- but for the
+    // compensated variant, which keeps one candidate and direct reads (its two-candidate
+    // and staged forms spill)
+    *shape_out = variant == HSS_KAHAN ? GridShape{1, 1, 0}
+                                      : GridShape{2, 1, model == HSS_CMP ? 1 : 0};
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return HSS_OK;

@@ -184,8 +184,9 @@ int hss_grid_semblance_plain(const hss_traces *traces, int model, const float *d
  * unstaged, and for CMP two staged) on the first launch slots of the caller's batch
  * (about 8192 workgroups) and keeping the fastest, a shape listed earlier (two candidates
  * per lane, one t0 tile, unstaged first) winning ties within 1%; the environment variable
- * HSS_AUTOTUNE=0 turns it off.  An untuned launch runs two candidates per lane (one for
- * HSS_KAHAN), one t0 tile, unstaged.  Results do not depend on the shape (bit for bit). */
+ * HSS_AUTOTUNE=0 turns it off.  An untuned launch runs two candidates per lane, one t0
+ * tile, staged for CMP (HSS_KAHAN: one candidate, unstaged).  Results do not depend on the
+ * shape (bit for bit). */
 int hss_grid_tuned_shape(int model, int w, int variant, int storage, int32_t *cpl,
                          int32_t *t0_tiles, int32_t *staged);
 /* hss_grid_tuned_shape's candidates per lane only. */
