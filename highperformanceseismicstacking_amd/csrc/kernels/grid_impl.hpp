@@ -100,10 +100,18 @@ __device__ __forceinline__ void merge_bests(float best, float bstack, int bidx, 
 // autotuner picks one.)
 template <int VARIANT, int CPL>
 constexpr int grid_min_waves() { return VARIANT == HSS_KAHAN && CPL == 1 ? 7 : 1; }
+// The grid kernel's budget: OCT, which holds both trace loops (with and without the kept
+// offset terms), at five waves per SIMD -- the loop without them fits as before (93
+// registers), the one with them spills a few run-level values; at the compiler's own
+// choice (107 registers, four waves) the reference-order line ran 6% slower.
+template <int MODEL, int VARIANT, int CPL>
+constexpr int grid_waves_budget() {
+    return MODEL == HSS_OCT && VARIANT != HSS_KAHAN ? 5 : grid_min_waves<VARIANT, CPL>();
+}
 
 template <int MODEL, int W, int VARIANT, int STORAGE, int CPL>
 __global__ __launch_bounds__(kGridThreads)
-__attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void grid_semblance_kernel(
+__attribute__((amdgpu_waves_per_eu(grid_waves_budget<MODEL, VARIANT, CPL>(), 8))) void grid_semblance_kernel(
     GridArgs a) {
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
@@ -129,6 +137,29 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
+    // OCT: keep the offset-dependent traveltime terms across runs of traces with one half
+    // offset (evaluate_n's OCT_KEEP, the same bits) when the midpoint's list is grouped by
+    // offset -- runs of four traces or more on average (the C4 line's grid search: 1849 ->
+    // 850 ms); in the reference's (m, h) order, where runs are single traces, keeping only
+    // adds work.  Counted once per workgroup over the list (wave-uniform).
+    bool keep = false;
+    if constexpr (MODEL == HSS_OCT) {
+        unsigned changes = 0;
+        for (int j0 = 0; j0 < e.cnt; j0 += kWave) {
+            const int j = j0 + lane;
+            bool change = false;
+            if (j < e.cnt) {
+                const int ti = e.midx ? e.midx[e.beg + j] : e.beg + j;
+                change = j == 0;
+                if (j > 0) {
+                    const int tp = e.midx ? e.midx[e.beg + j - 1] : e.beg + j - 1;
+                    change = __float_as_uint(e.hoff[ti]) != __float_as_uint(e.hoff[tp]);
+                }
+            }
+            changes += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(change));
+        }
+        keep = 4u * changes <= (unsigned)e.cnt;
+    }
 
     for (int q0 = grp * CPL; q0 < a.P; q0 += ngroups * CPL) {
         Candidate k[CPL];
@@ -140,8 +171,12 @@ __attribute__((amdgpu_waves_per_eu(grid_min_waves<VARIANT, CPL>(), 8))) void gri
         }
         float S[CPL], stk[CPL];
         unsigned rej_q = 0;
-        evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, grid_segments<STORAGE>()>(
-            e, t0, t0sq, k, on, 0.0f, active, seg_lds[wave], S, stk, &rej_q);
+        if (MODEL == HSS_OCT && keep)
+            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, grid_segments<STORAGE>(), true>(
+                e, t0, t0sq, k, on, 0.0f, active, seg_lds[wave], S, stk, &rej_q);
+        else
+            evaluate_n<MODEL, W, VARIANT, STORAGE, false, CPL, grid_segments<STORAGE>()>(
+                e, t0, t0sq, k, on, 0.0f, active, seg_lds[wave], S, stk, &rej_q);
         if (active) {
             rejected += rej_q;
 #pragma unroll
