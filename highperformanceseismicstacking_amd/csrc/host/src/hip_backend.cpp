@@ -394,7 +394,18 @@ void HipLinearSearchAlgorithm::selectBatch(const std::vector<float> &m0s) {
         oct.npairs = static_cast<int32_t>(dOctPairs.size() / (2 * sizeof(float)));
         auto lists = select_oct_lists(line, m0s, gather->getApm(), traveltime->getReferenceHalfoffset(), oct,
                                       dMask, stream(), [](size_t, hss_oct_candidates &) {});
-        for (size_t g = 0; g < m0s.size(); g++) batch.addIndexList(m0s[g], lists[g]);
+        // the non-exact variants sum the traces grouped by half offset: the grid kernel
+        // then keeps the offset-dependent half of the OCT traveltime across each group
+        // (the order changes the sums' rounding only; the exact variant keeps the
+        // reference's order)
+        const bool grouped = BackendOptions::get().variant != HSS_EXACT;
+        const std::vector<float> &h = line.halfoffsets();
+        for (size_t g = 0; g < m0s.size(); g++) {
+            if (grouped)
+                std::stable_sort(lists[g].begin(), lists[g].end(),
+                                 [&](int32_t a, int32_t b) { return h[a] < h[b]; });
+            batch.addIndexList(m0s[g], lists[g]);
+        }
     }
     batch.upload(stream());
 }

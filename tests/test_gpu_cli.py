@@ -278,6 +278,55 @@ def test_linear_search_zocrs_and_oct(tmp_path):
         assert np.array_equal(got.view(np.uint32), rres.view(np.uint32)), m
 
 
+def test_linear_search_oct_fast_groups_traces_by_offset(tmp_path):
+    """The non-exact variants hand the OCT grid each midpoint's traces grouped by half
+    offset (the kernel then keeps the offset-dependent half of the traveltime across each
+    group; only the sums' rounding changes): against the oracle in the reference's order,
+    the coherence is within 1e-5, and where the oracle's best candidate has no tie within
+    1e-6 the program reports the same velocity and slope and a stack within 1e-5."""
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 10)
+    line = synth.Line(line.samples[:, :251].copy(), line.sx, line.gx, line.scalco, 251, line.dt_us,
+                      line.cdp)
+    path = str(tmp_path / "o.su")
+    synth.write_su(path, line)
+    tau = synth.tau_index(0.02, line.dt_us)
+    apm, h0 = 30.0, 100.0
+    G = [8, 4]
+    lo, hi = [1500.0, -5e-4], [4500.0, 5e-4]
+    args = ["--aph", "5000", "--apm", apm, "--azimuth", "90", "--input", path, "--output",
+            str(tmp_path), "--tau", "0.02", "--traveltime", "oct", "--lower-bounds", *lo,
+            "--upper-bounds", *hi, "--h0", h0, "--granularity", *G, "--variant", "fast"]
+    outdir, _ = run("single_host_linear_search", args, tmp_path)
+    res = read_results(outdir, ["coherence", "stack", "velocity", "slope"], line.ns)
+    params = O.build_parameter_array(O.OCT, lo, grid_steps(lo, hi, G), G)
+    pairs = O.oct_selection_samples(lo[0], hi[0], lo[1], hi[1], 1024)
+    checked = 0
+    for k, m in enumerate(cdp_list(line)):
+        if m == 0.0:
+            continue
+        mask = O.select_traces(O.OCT, line.mid, line.hoff, m, apm=apm, h0=h0, ns=line.ns, dt=line.dt,
+                               pairs=pairs, npairs=1024)
+        sel = np.flatnonzero(mask)
+        got = np.stack([res[n][1][k] for n in ["coherence", "stack", "velocity", "slope"]])
+        if sel.size == 0:
+            assert not got.any()
+            continue
+        og = O.Gather(line.samples[sel], line.mid[sel], line.hoff[sel], line.dt, tau, m0=float(m),
+                      h0=h0, apm=apm)
+        rres, rS, _, _, _ = O.grid_f32(O.OCT, og, params, full=True)
+        assert np.all(np.abs(got[0] - rres[0]) <= 1e-5 * np.abs(rres[0]) + 1e-6), m
+        best = np.nan_to_num(rres[0], nan=-1.0)
+        near = (np.nan_to_num(rS, nan=-2.0) >= (best - 1e-6)[:, None]).sum(axis=1)
+        single = near == 1
+        assert np.array_equal(got[2][single].view(np.uint32), rres[2][single].view(np.uint32)), m
+        assert np.array_equal(got[3][single].view(np.uint32), rres[3][single].view(np.uint32)), m
+        scale = max(float(np.abs(rres[1]).max()), 1e-30)
+        assert np.all(np.abs(got[1][single] - rres[1][single]) <= 1e-5 * scale), m
+        checked += int(single.sum())
+    assert checked > 0
+
+
 def test_oct_selection_in_chunks_changes_nothing(tmp_path):
     """The host layer selects OCT traces in chunks of midpoints under a mask budget
     (HSS_SELECT_MASK_BYTES); chunks of three midpoints give the linear search and the
