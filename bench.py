@@ -68,6 +68,9 @@ def parse():
                     help="launch the step directly instead of replaying captured hipGraphs")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline step (no comparison legs, no other workloads)")
+    ap.add_argument("--grid-leg", action="store_true",
+                    help="c3 / c4: also time a grid search over the DE line's midpoints and "
+                         "traces (config.grid_search; the default run's extras include it)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the timed steps, run one plain device-to-device copy of this "
                          "many MiB (a known-size reference for rocprofv3 FETCH/WRITE counters)")
