@@ -15,7 +15,8 @@ FP32 EXACT kernel (the reference's kernel structure); the precision path (FP16 s
 compensated window sums, then 10 Gauss-Newton steps of the adjoint polish); the C2 line
 (500 midpoints, 1024 velocities); the ZO-CRS DE line time (C3) and the OCT DE line time
 (C4), each with a grid search over the same midpoints and traces (a candidate grid across
-the DE bounds); and a device-to-device copy as the measured HBM reference.
+the DE bounds) and the stretch-free re-stack of the DE winners; and a device-to-device copy
+as the measured HBM reference.
 
 Roofline of the dominant kernel (the grid launch; HIP events on its stream, inside the
 timed region): the kernel is bound by its FP32 vector instruction stream, so `achieved`
@@ -71,6 +72,9 @@ def parse():
     ap.add_argument("--grid-leg", action="store_true",
                     help="c3 / c4: also time a grid search over the DE line's midpoints and "
                          "traces (config.grid_search; the default run's extras include it)")
+    ap.add_argument("--sf-leg", action="store_true",
+                    help="c3 / c4: also time the stretch-free re-stack of the DE winners "
+                         "(config.stretch_free; the default run's extras include it)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the timed steps, run one plain device-to-device copy of this "
                          "many MiB (a known-size reference for rocprofv3 FETCH/WRITE counters)")
@@ -449,17 +453,19 @@ def other_workloads(args):
         de_args = argparse.Namespace(**vars(args))
         de_args.workload, de_args.storage, de_args.variant = wl, storage, "fast"
         de_args.steps, de_args.warmup, de_args.calibrate_copy_mb = 3, 1, 0
-        de_args.grid_leg = True
+        de_args.grid_leg = de_args.sf_leg = True
         de = run_de(de_args)
         key = "zocrs_de" if wl == "c3" else "oct_de"
         progress(f"{wl} line {de['config']['line_time_s']:.3f} s, grid "
-                 f"{de['config']['grid_search']['launch_ms']:.1f} ms")
+                 f"{de['config']['grid_search']['launch_ms']:.1f} ms, stretch-free "
+                 f"{de['config']['stretch_free']['launch_ms']:.1f} ms")
         out[key] = {"workload": de["config"]["workload"],
                     "line_time_s": de["config"]["line_time_s"],
                     "de_launch_ms": de["config"]["de_launch_ms"],
                     "polish_launch_ms": de["config"]["polish_launch_ms"],
                     "evals_per_s": de["value"],
-                    "grid_search": de["config"]["grid_search"]}
+                    "grid_search": de["config"]["grid_search"],
+                    "stretch_free": de["config"]["stretch_free"]}
         torch.cuda.empty_cache()
     return out
 
@@ -697,7 +703,51 @@ def run_de(args):
             "launch_ms": grid_ms,
             "evals_per_s": int(mp.n) * line.ns * P / (grid_ms * 1e-3),
         }
+    if getattr(args, "sf_leg", False):
+        out["config"]["stretch_free"] = stretch_free_leg(args, line, dl, mp, model, res, m0s, h0,
+                                                         apm, variant, stream)
     calibration_copy(args.calibrate_copy_mb)
+    return out
+
+
+def stretch_free_leg(args, line, dl, mp, model, res, m0s, h0, apm, variant, stream):
+    """The stretch-free re-stack (21 shifts, -10..10) of a DE line's winners over the same
+    midpoints.  ZO-CRS keeps the DE ranges; OCT selects each midpoint's traces with the
+    winners' per-sample (velocity, slope), as the host layer does, grouped by half offset
+    for the non-exact variants."""
+    import numpy as np
+    import torch
+
+    from highperformanceseismicstacking_amd import EXACT, OCT, capi
+    np_ = res.shape[1] - 2
+    vpar = res[:, 2:2 + np_, :].contiguous()
+    nvals = np.arange(-10, 11, dtype=np.float32)
+    smp = mp
+    counts = None
+    if model == OCT:
+        mask = capi.select_traces(dl, OCT, m0s, apm=apm, h0=h0, oct_mode=capi.OCT_PER_SAMPLE,
+                                  vparams=vpar)
+        torch.cuda.synchronize()
+        m = mask.cpu().numpy().astype(bool)
+        del mask
+        idx = [np.flatnonzero(m[g]) for g in range(len(m0s))]
+        if variant != EXACT:
+            idx = [x[np.argsort(line.hoff[x], kind="stable")] for x in idx]
+        counts = np.array([len(x) for x in idx])
+        begins = np.r_[0, np.cumsum(counts)[:-1]]
+        smp = capi.DeviceMidpoints(begins, counts, m0s, index=np.concatenate(idx), h0=h0, apm=apm,
+                                   order="balanced")
+
+    def sf():
+        capi.stretch_free(dl, smp, model, vpar, nvals, variant=variant, stream=stream)
+
+    sf()
+    torch.cuda.synchronize()
+    sf_ms = time_region(sf, max(2, args.steps), 1, stream)
+    out = {"shifts": int(nvals.size), "launch_ms": sf_ms,
+           "evals_per_s": int(smp.n) * line.ns * int(nvals.size) / (sf_ms * 1e-3)}
+    if counts is not None:
+        out["traces_per_midpoint"] = f"{int(counts.min())}-{int(counts.max())} (mean {counts.mean():.0f})"
     return out
 
 

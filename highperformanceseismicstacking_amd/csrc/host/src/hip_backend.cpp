@@ -271,6 +271,16 @@ static std::vector<std::vector<int32_t>> select_oct_lists(const HipLine &line, c
     return lists;
 }
 
+// OCT trace lists for the non-exact variants are summed grouped by half offset: the
+// kernels then keep the offset-dependent half of the OCT traveltime across each group
+// (the order changes the sums' rounding only; the exact variant keeps the reference's
+// order, oct/*.cu's trace loops over the selected (m, h) traces).
+static void group_by_halfoffset(std::vector<int32_t> &list, const HipLine &line) {
+    if (BackendOptions::get().variant == HSS_EXACT) return;
+    const std::vector<float> &h = line.halfoffsets();
+    std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) { return h[a] < h[b]; });
+}
+
 // Launch order of a batch's midpoints.  The kernels hand (midpoint, tile) workgroups to
 // the 8 XCDs in contiguous eighths of the launch (logical_block in the kernels), and the
 // folds of a line differ; sorting the midpoints by work and dealing them to 8 bins in
@@ -394,16 +404,8 @@ void HipLinearSearchAlgorithm::selectBatch(const std::vector<float> &m0s) {
         oct.npairs = static_cast<int32_t>(dOctPairs.size() / (2 * sizeof(float)));
         auto lists = select_oct_lists(line, m0s, gather->getApm(), traveltime->getReferenceHalfoffset(), oct,
                                       dMask, stream(), [](size_t, hss_oct_candidates &) {});
-        // the non-exact variants sum the traces grouped by half offset: the grid kernel
-        // then keeps the offset-dependent half of the OCT traveltime across each group
-        // (the order changes the sums' rounding only; the exact variant keeps the
-        // reference's order)
-        const bool grouped = BackendOptions::get().variant != HSS_EXACT;
-        const std::vector<float> &h = line.halfoffsets();
         for (size_t g = 0; g < m0s.size(); g++) {
-            if (grouped)
-                std::stable_sort(lists[g].begin(), lists[g].end(),
-                                 [&](int32_t a, int32_t b) { return h[a] < h[b]; });
+            group_by_halfoffset(lists[g], line);
             batch.addIndexList(m0s[g], lists[g]);
         }
     }
@@ -580,15 +582,7 @@ void HipDifferentialEvolutionAlgorithm::selectBatch(const std::vector<float> &m0
                                          traveltime->getReferenceHalfoffset(), &oct,
                                          dMask.as<uint8_t>(), stream()));
             std::vector<int32_t> list = masks_to_lists(dMask, 1, ntr, stream())[0];
-            // the non-exact variants sum the traces grouped by half offset: the kernels
-            // then compute the offset-dependent half of the OCT traveltime once per group
-            // (the order changes the sums' rounding only; the exact variant keeps the
-            // reference's order)
-            if (BackendOptions::get().variant != HSS_EXACT) {
-                const std::vector<float> &h = line.halfoffsets();
-                std::stable_sort(list.begin(), list.end(),
-                                 [&](int32_t a, int32_t b) { return h[a] < h[b]; });
-            }
+            group_by_halfoffset(list, line);
             batch.addIndexList(m0s[g], list);
         }
     }
@@ -755,7 +749,10 @@ void HipStretchFreeAlgorithm::selectBatch(const std::vector<float> &m0s) {
                                       dMask, stream(), [&](size_t first, hss_oct_candidates &c) {
                                           c.d_vparams = vparams + first * perMidpoint;
                                       });
-        for (size_t g = 0; g < m0s.size(); g++) batch.addIndexList(m0s[g], lists[g]);
+        for (size_t g = 0; g < m0s.size(); g++) {
+            group_by_halfoffset(lists[g], line);
+            batch.addIndexList(m0s[g], lists[g]);
+        }
     }
     batch.upload(stream());
 }

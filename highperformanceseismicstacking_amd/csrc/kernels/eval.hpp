@@ -375,7 +375,7 @@ __device__ __forceinline__ void evaluate_n(const EvalCtx &e, float t0, float t0s
 
 // One candidate per lane.
 template <int MODEL, int W, int VARIANT, int STORAGE, bool SHIFT, bool SEG = (STORAGE != HSS_F32),
-          bool FACTORED16 = true>
+          bool OCT_KEEP = false, bool FACTORED16 = true>
 __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
                                          const Candidate &k, float shift, bool active,
                                          float *seg_generic, float *S, float *stack,
@@ -383,7 +383,7 @@ __device__ __forceinline__ void evaluate(const EvalCtx &e, float t0, float t0sq,
     const Candidate kk[1] = {k};
     const bool on[1] = {true};
     float s1[1], t1[1];
-    evaluate_n<MODEL, W, VARIANT, STORAGE, SHIFT, 1, SEG, false, FACTORED16>(
+    evaluate_n<MODEL, W, VARIANT, STORAGE, SHIFT, 1, SEG, OCT_KEEP, FACTORED16>(
         e, t0, t0sq, kk, on, shift, active, seg_generic, s1, t1, rejected);
     *S = s1[0];
     *stack = t1[0];
@@ -408,6 +408,30 @@ __device__ __forceinline__ EvalCtx midpoint_ctx(const EvalCtx &base, const int *
     e.cnt = mcnt[g];
     e.m0 = mm0[g];
     return e;
+}
+
+// Whether midpoint e's trace list comes in runs of one half offset long enough for
+// evaluate_n's OCT_KEEP to pay: four traces or more per run on average (lists grouped by
+// offset; in the reference's (m, h) order runs are single traces, where keeping only adds
+// work).  Runs are told apart by the offset's bit pattern, the test OCT_KEEP itself makes.
+// Every lane of the wave must call it; the result is wave-uniform.
+__device__ __forceinline__ bool oct_offset_runs_long(const EvalCtx &e) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned changes = 0;
+    for (int j0 = 0; j0 < e.cnt; j0 += kWave) {
+        const int j = j0 + lane;
+        bool change = false;
+        if (j < e.cnt) {
+            const int ti = e.midx ? e.midx[e.beg + j] : e.beg + j;
+            change = j == 0;
+            if (j > 0) {
+                const int tp = e.midx ? e.midx[e.beg + j - 1] : e.beg + j - 1;
+                change = __float_as_uint(e.hoff[ti]) != __float_as_uint(e.hoff[tp]);
+            }
+        }
+        changes += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(change));
+    }
+    return 4u * changes <= (unsigned)e.cnt;
 }
 
 }  // namespace hss

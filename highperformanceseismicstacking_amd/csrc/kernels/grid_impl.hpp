@@ -138,28 +138,10 @@ __attribute__((amdgpu_waves_per_eu(grid_waves_budget<MODEL, VARIANT, CPL>(), 8))
     int bidx = -1;
     unsigned long long rejected = 0;
     // OCT: keep the offset-dependent traveltime terms across runs of traces with one half
-    // offset (evaluate_n's OCT_KEEP, the same bits) when the midpoint's list is grouped by
-    // offset -- runs of four traces or more on average (the C4 line's grid search: 1849 ->
-    // 850 ms); in the reference's (m, h) order, where runs are single traces, keeping only
-    // adds work.  Counted once per workgroup over the list (wave-uniform).
+    // offset when the midpoint's list is grouped by offset (the C4 line's grid search: 1849
+    // -> 850 ms).  Counted once per workgroup over the list (wave-uniform).
     bool keep = false;
-    if constexpr (MODEL == HSS_OCT) {
-        unsigned changes = 0;
-        for (int j0 = 0; j0 < e.cnt; j0 += kWave) {
-            const int j = j0 + lane;
-            bool change = false;
-            if (j < e.cnt) {
-                const int ti = e.midx ? e.midx[e.beg + j] : e.beg + j;
-                change = j == 0;
-                if (j > 0) {
-                    const int tp = e.midx ? e.midx[e.beg + j - 1] : e.beg + j - 1;
-                    change = __float_as_uint(e.hoff[ti]) != __float_as_uint(e.hoff[tp]);
-                }
-            }
-            changes += (unsigned)__builtin_popcountll(__builtin_amdgcn_ballot_w64(change));
-        }
-        keep = 4u * changes <= (unsigned)e.cnt;
-    }
+    if constexpr (MODEL == HSS_OCT) keep = oct_offset_runs_long(e);
 
     for (int q0 = grp * CPL; q0 < a.P; q0 += ngroups * CPL) {
         Candidate k[CPL];

@@ -42,8 +42,18 @@ struct SfArgs {
     unsigned long long *not_used;
 };
 
+// The budget of OCT's kernel, which holds both trace loops (with and without the kept
+// offset terms): six waves per SIMD for windows of up to 11 samples, where both loops fit
+// without spills (the C4 line's w = 11: the loop in the reference's order would otherwise
+// take 81 registers, five waves); wider and generic windows at the compiler's own count.
+template <int MODEL, int W, int VARIANT>
+constexpr int sf_min_waves() {
+    return MODEL == HSS_OCT && VARIANT != HSS_KAHAN && W > 0 && W <= 11 ? 6 : 1;
+}
+
 template <int MODEL, int W, int VARIANT, int STORAGE>
-__global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
+__global__ __launch_bounds__(kGridThreads)
+__attribute__((amdgpu_waves_per_eu(sf_min_waves<MODEL, W, VARIANT>(), 8))) void sf_semblance_kernel(SfArgs a) {
     constexpr int NP = MODEL == HSS_CMP ? 1 : (MODEL == HSS_ZOCRS ? 3 : 2);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -62,6 +72,14 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
+    // OCT: keep the offset-dependent traveltime terms across runs of traces with one half
+    // offset when the midpoint's list is grouped by offset (wave-uniform); each lane's
+    // candidate comes from its own sample, and the kept terms are per lane all the same
+    bool keep = false;
+    if constexpr (MODEL == HSS_OCT) keep = oct_offset_runs_long(e);
+    // a BF16 line stored without scales sums its windows as FP32 windows; both choices are
+    // made outside the trace loop, one code path each
+    const bool unscaled16 = STORAGE == HSS_BF16 && e.scale == nullptr;
     for (int q = wave; q < a.nN; q += kGridWaves) {
         const int n = (int)a.nvals[q];
         const int sn = s - n;
@@ -79,14 +97,23 @@ __global__ __launch_bounds__(kGridThreads) void sf_semblance_kernel(SfArgs a) {
         const float t0n = (float)sc * e.dt;
         float S = 0.0f, stk = 0.0f;
         unsigned rej = 0;
-        // a BF16 line stored without scales sums its windows as FP32 windows (one form per
-        // code path, chosen outside the trace loop)
-        if (STORAGE == HSS_BF16 && e.scale == nullptr)
-            evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments, false>(
-                e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
-        else
-            evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments>(
-                e, t0n, t0n * t0n, k, (float)n * e.dt, valid, seg_lds[wave], &S, &stk, &rej);
+        auto eval = [&](auto keep_c, auto factored_c) {
+            evaluate<MODEL, W, VARIANT, STORAGE, true, kSfSegments, decltype(keep_c)::value,
+                     decltype(factored_c)::value>(e, t0n, t0n * t0n, k, (float)n * e.dt, valid,
+                                                  seg_lds[wave], &S, &stk, &rej);
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        if (MODEL == HSS_OCT && keep) {
+            if (unscaled16)
+                eval(T{}, F{});
+            else
+                eval(T{}, T{});
+        } else if (unscaled16) {
+            eval(F{}, F{});
+        } else {
+            eval(F{}, T{});
+        }
         if (!valid) {
             S = 0.0f;
             stk = 0.0f;

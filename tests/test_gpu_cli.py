@@ -327,6 +327,61 @@ def test_linear_search_oct_fast_groups_traces_by_offset(tmp_path):
     assert checked > 0
 
 
+def test_stretch_free_oct_fast_groups_traces_by_offset(tmp_path):
+    """The non-exact variants hand the OCT stretch-free re-stack each midpoint's traces
+    grouped by half offset, as they do the grid: against the oracle over the per-sample
+    selection in that order, the coherence is within 1e-5 and, where the oracle's best
+    shift has no rival within 1e-6, the program picks the same shift."""
+    line, cfg = synth.workload("c4")
+    line = line.subset(0, 16 * 10)
+    line = synth.Line(line.samples[:, :251].copy(), line.sx, line.gx, line.scalco, 251, line.dt_us,
+                      line.cdp)
+    path = str(tmp_path / "o.su")
+    synth.write_su(path, line)
+    tau = synth.tau_index(0.02, line.dt_us)
+    apm, h0 = 30.0, 100.0
+    lo, hi = [1500.0, -5e-4], [4500.0, 5e-4]
+    common = ["--aph", "5000", "--apm", apm, "--azimuth", "90", "--input", path, "--tau", "0.02",
+              "--traveltime", "oct", "--lower-bounds", *lo, "--upper-bounds", *hi, "--h0", h0]
+    os.makedirs(tmp_path / "ls", exist_ok=True)
+    os.makedirs(tmp_path / "sf", exist_ok=True)
+    ls, _ = run("single_host_linear_search", common + ["--output", str(tmp_path / "ls"),
+                                                       "--granularity", 8, 4, "--variant", "exact"],
+                tmp_path)
+    vel, slope = os.path.join(ls, "velocity.su"), os.path.join(ls, "slope.su")
+    sf, _ = run("single_host_stretch_free",
+                common + ["--output", str(tmp_path / "sf"), "--stack-datafiles", vel, slope,
+                          "--variant", "fast"], tmp_path)
+    res = read_results(sf, ["coherence", "stack", "n"], line.ns)
+    _, vdata = synth.read_su(vel, line.ns)
+    _, sdata = synth.read_su(slope, line.ns)
+    nvals = np.arange(-10, 11, dtype=np.float32)
+    checked = 0
+    for k, m in enumerate(cdp_list(line)):
+        if m == 0.0:
+            continue
+        v = vdata[k].astype(np.float32)
+        pr = np.stack([f32(4.0) / (v * v), sdata[k]], axis=1).astype(np.float32).ravel()
+        mask = O.select_traces(O.OCT, line.mid, line.hoff, m, apm=apm, h0=h0, ns=line.ns, dt=line.dt,
+                               pairs=pr, npairs=1, per_sample=True)
+        sel = np.flatnonzero(mask)
+        got = np.stack([res[n][1][k] for n in ["coherence", "stack", "n"]])
+        if sel.size == 0:
+            assert not got.any()
+            continue
+        sel = sel[np.argsort(line.hoff[sel], kind="stable")]
+        og = O.Gather(line.samples[sel], line.mid[sel], line.hoff[sel], line.dt, tau, m0=float(m),
+                      h0=h0, apm=apm)
+        vp = np.stack([v, sdata[k]]).astype(np.float32)
+        r, rS, _ = O.stretch_free_f32(O.OCT, og, vp.reshape(-1), nvals, full=True)
+        assert np.all(np.abs(got[0] - r[0]) <= 1e-5 * np.abs(r[0]) + 1e-6), m
+        near = (np.nan_to_num(rS, nan=-2.0) >= (r[0] - 1e-6)[:, None]).sum(axis=1)
+        single = near == 1
+        assert np.array_equal(got[2][single], r[2][single]), m
+        checked += int(single.sum())
+    assert checked > 0
+
+
 def test_oct_selection_in_chunks_changes_nothing(tmp_path):
     """The host layer selects OCT traces in chunks of midpoints under a mask budget
     (HSS_SELECT_MASK_BYTES); chunks of three midpoints give the linear search and the

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from highperformanceseismicstacking_amd import OCT, EXACT, FAST, F32, F16, capi, synth
+from highperformanceseismicstacking_amd import OCT, EXACT, FAST, F32, F16, BF16, capi, synth
 from helpers import zocrs_window
 
 pytestmark = pytest.mark.gpu
@@ -145,3 +145,67 @@ def test_de_fast_in_offset_order_reports_semblance_of_its_parameters():
             prm = [np.float32(4.0) / (v * v), res[g, 3, s]]
             S, _, _, _ = O.eval_f32(OCT, og, s, prm)
             assert abs(S - res[g, 0, s]) <= 1e-5 * abs(S) + 1e-6, (g, s, S, res[g, 0, s])
+
+
+def _sf_case(h0, storage):
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(h0)
+    tau = synth.tau_index(0.02, 4000)
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    rng = np.random.default_rng(17)
+    vp = np.stack([rng.uniform(lo[0], hi[0], (len(m0), line.ns)),
+                   rng.uniform(lo[1], hi[1], (len(m0), line.ns))], axis=1).astype(np.float32)
+    vpar = torch.from_numpy(vp).cuda()
+    nvals = np.arange(-10, 11, dtype=np.float32)
+    return line, lists, begins, counts, m0, h0, apm, tau, dl, vp, vpar, nvals
+
+
+@pytest.mark.parametrize("storage", [F16, BF16])
+@pytest.mark.parametrize("h0", [0.0, 150.0])
+def test_stretch_free_exact_in_offset_order_equal_oracle(storage, h0):
+    """Stretch-free over offset-grouped lists keeps the OCT offset terms per lane (each
+    lane's parameters come from its own shifted sample): with 16-bit storage (BF16 summed
+    in the FP32 steps, F16 factored) the exact variant is the oracle's bit for bit on the
+    decoded samples, and the rejection count is the one of the same traces in line order
+    (one aperture decision per trace and parameter pair, whatever the order)."""
+    import torch
+    line, lists, begins, counts, m0, h0, apm, tau, dl, vp, vpar, nvals = _sf_case(h0, storage)
+    grouped = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    natural = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate([np.sort(x) for x in lists]),
+                                   h0=h0, apm=apm)
+    res, sem, stk, nu = capi.stretch_free(dl, grouped, OCT, vpar, nvals, variant=EXACT, full=True,
+                                          not_used=True)
+    _, _, _, nu_nat = capi.stretch_free(dl, natural, OCT, vpar, nvals, variant=EXACT, not_used=True)
+    torch.cuda.synchronize()
+    res, sem, stk = res.cpu().numpy(), sem.cpu().numpy(), stk.cpu().numpy()
+    nu, nu_nat = nu.cpu().numpy(), nu_nat.cpu().numpy()
+    for g in range(len(m0)):
+        og = _gather(dl.decoded, line, lists[g], tau, float(m0[g]), h0, apm)
+        r, rS, rT = O.stretch_free_f32(OCT, og, vp[g].reshape(-1), nvals, full=True)
+        assert np.array_equal(_bits(res[g]), _bits(r)), g
+        assert np.array_equal(_bits(sem[g]), _bits(rS)), g
+        assert np.array_equal(_bits(stk[g]), _bits(rT)), g
+        assert nu[g] > 0 and nu[g] == nu_nat[g], g
+
+
+def test_stretch_free_fast_in_offset_order_within_tolerance():
+    """FAST stretch-free over offset-grouped lists (how the host layer runs it): every
+    shift's coherence within 1e-5 of the oracle's over the same list, and the picked shift
+    the oracle's wherever its best has no rival within 1e-6."""
+    import torch
+    line, lists, begins, counts, m0, h0, apm, tau, dl, vp, vpar, nvals = _sf_case(150.0, F32)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    res, sem, _, _ = capi.stretch_free(dl, mp, OCT, vpar, nvals, variant=FAST, full=True)
+    torch.cuda.synchronize()
+    res, sem = res.cpu().numpy(), sem.cpu().numpy()
+    checked = 0
+    for g in range(len(m0)):
+        og = _gather(line.samples, line, lists[g], tau, float(m0[g]), h0, apm)
+        r, rS, _ = O.stretch_free_f32(OCT, og, vp[g].reshape(-1), nvals, full=True)
+        a, b = np.nan_to_num(sem[g], nan=-2.0), np.nan_to_num(rS, nan=-2.0)
+        assert np.all(np.abs(a - b) <= 1e-5 * np.abs(b) + 1e-6), g
+        near = (b >= (r[0] - 1e-6)[:, None]).sum(axis=1)
+        single = near == 1
+        assert np.array_equal(res[g, 2][single], r[2][single]), g
+        checked += int(single.sum())
+    assert checked > 0
