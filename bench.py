@@ -144,7 +144,7 @@ def time_region(fn, steps, warmup, stream):
 
 
 def read_profile_figures(kernel, workload, storage):
-    """Figures per launch of `kernel` ("grid", "de") on `workload` from the committed
+    """Figures per launch of `kernel` ("grid", "de", "sf") on `workload` from the committed
     rocprofv3 summary profiles/traffic_<kernel>_<workload>_<storage>.txt: HBM bytes
     (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), LDS bytes and LDS bank-conflict
     cycles, and the kernel time the counters were taken at.  {} when none is committed."""
@@ -746,6 +746,10 @@ def stretch_free_leg(args, line, dl, mp, model, res, m0s, h0, apm, variant, stre
     sf_ms = time_region(sf, max(2, args.steps), 1, stream)
     out = {"shifts": int(nvals.size), "launch_ms": sf_ms,
            "evals_per_s": int(smp.n) * line.ns * int(nvals.size) / (sf_ms * 1e-3)}
+    prof = read_profile_figures("sf", args.workload, args.storage)
+    if "traffic_bytes_per_launch" in prof:
+        out["traffic"] = prof["traffic_bytes_per_launch"]
+        out["traffic_source"] = prof["source"]
     if counts is not None:
         out["traces_per_midpoint"] = f"{int(counts.min())}-{int(counts.max())} (mean {counts.mean():.0f})"
     return out
