@@ -275,10 +275,18 @@ static std::vector<std::vector<int32_t>> select_oct_lists(const HipLine &line, c
 // kernels then keep the offset-dependent half of the OCT traveltime across each group
 // (the order changes the sums' rounding only; the exact variant keeps the reference's
 // order, oct/*.cu's trace loops over the selected (m, h) traces).
+// The sort key is a total order on the offset's bit pattern (ascending for ordinary
+// values), so equal patterns -- the kernels' test for one run -- sit together and a NaN
+// offset cannot break the ordering.
 static void group_by_halfoffset(std::vector<int32_t> &list, const HipLine &line) {
     if (BackendOptions::get().variant == HSS_EXACT) return;
     const std::vector<float> &h = line.halfoffsets();
-    std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) { return h[a] < h[b]; });
+    auto key = [&](int32_t i) {
+        uint32_t u;
+        std::memcpy(&u, &h[i], sizeof u);
+        return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    };
+    std::stable_sort(list.begin(), list.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
 }
 
 // Launch order of a batch's midpoints.  The kernels hand (midpoint, tile) workgroups to

@@ -209,3 +209,40 @@ def test_stretch_free_fast_in_offset_order_within_tolerance():
         assert np.array_equal(res[g, 2][single], r[2][single]), g
         checked += int(single.sum())
     assert checked > 0
+
+
+def test_grid_and_stretch_free_mixed_orders_in_one_launch():
+    """Each workgroup decides from its own midpoint's list whether to keep the offset
+    terms: a launch whose midpoints alternate between offset-grouped and line-order lists
+    runs both trace loops side by side, and the exact variant matches the oracle on every
+    midpoint bit for bit (grid search and stretch-free)."""
+    import torch
+    line, lists, begins, counts, m0, lo, hi, h0, apm = _case(150.0)
+    lists = [x if g % 2 == 0 else np.sort(x) for g, x in enumerate(lists)]
+    for g, x in enumerate(lists):
+        h = line.hoff[x].view(np.uint32)
+        runs = 1 + np.count_nonzero(h[1:] != h[:-1])
+        assert (4 * runs <= len(x)) == (g % 2 == 0), g
+    tau = synth.tau_index(0.02, 4000)
+    Gv, Gs = 16, 5
+    steps = [float(np.float32(np.float32(hi[k] - lo[k]) / np.float32(G)))
+             for k, G in enumerate((Gv, Gs))]
+    params = capi.build_parameter_array(OCT, lo, steps, [Gv, Gs])
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=F16)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    res, sem, _, _ = capi.grid_semblance(dl, mp, OCT, params, Gv * Gs, variant=EXACT, full=True)
+    vpar = res[:, 2:4, :].contiguous()
+    nvals = np.arange(-4, 5, dtype=np.float32)
+    sf, ssem, _, _ = capi.stretch_free(dl, mp, OCT, vpar, nvals, variant=EXACT, full=True)
+    torch.cuda.synchronize()
+    res, sem, sf, ssem = res.cpu().numpy(), sem.cpu().numpy(), sf.cpu().numpy(), ssem.cpu().numpy()
+    vp = vpar.cpu().numpy()
+    pnp = params.cpu().numpy()
+    for g in range(len(m0)):
+        og = _gather(dl.decoded, line, lists[g], tau, float(m0[g]), h0, apm)
+        rres, rS, _, _, _ = O.grid_f32(OCT, og, pnp, full=True)
+        assert np.array_equal(_bits(res[g]), _bits(rres)), g
+        assert np.array_equal(_bits(sem[g]), _bits(rS)), g
+        r_sf, r_ss, _ = O.stretch_free_f32(OCT, og, vp[g].reshape(-1), nvals, full=True)
+        assert np.array_equal(_bits(sf[g]), _bits(r_sf)), g
+        assert np.array_equal(_bits(ssem[g]), _bits(r_ss)), g
