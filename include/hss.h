@@ -87,7 +87,12 @@ typedef struct hss_traces {
 /* The midpoints to process and the traces each one uses.  Midpoint g uses traces
  * index ? index[begin[g] .. begin[g]+count[g]) : [begin[g], begin[g]+count[g]),
  * i.e. the reference's per-midpoint FILT_* compaction (ComputeAlgorithm.cpp:107-153)
- * expressed as ranges into the resident line.
+ * expressed as ranges into the resident line.  The list's order is the order of the
+ * sums: HSS_EXACT equals the reference bit for bit when the list is in line order.  The
+ * OCT kernels (grid, DE, stretch-free) keep the offset-dependent half of the traveltime
+ * while consecutive traces share a half offset (the same bits as computing it again), so
+ * lists grouped by half offset run 2-2.3x faster; the non-exact variants of the host
+ * classes pass them so, which changes the sums' rounding only.
  *
  * `order` (no counterpart in the reference; NULL = 0, 1, ..., n-1) is the order in which
  * a launch takes up the midpoints: launch slot j works on midpoint order[j].  It changes
