@@ -108,8 +108,12 @@ __device__ __forceinline__ int oct_mh_dual(float h, float h0, float t0, float m0
     return kNoError;
 }
 
-__device__ __forceinline__ int oct_time_dual(float h, float h0, float t0, float m0, float m,
-                                             Dual<2> mh, Dual<2> c, Dual<2> slope, Dual<2> *out) {
+// OCT traveltime from the displaced midpoint (oct/common.cuh:7-94) as duals, in its two
+// parts: th and the coefficient ah of (m - mh), which depend on the trace's half offset
+// only; the traveltime at midpoint m is th + ah * (m - mh) (oct_time_from_terms).
+__device__ __forceinline__ int oct_time_terms_dual(float h, float h0, float t0, float m0,
+                                                   Dual<2> mh, Dual<2> c, Dual<2> slope,
+                                                   Dual<2> *th_out, Dual<2> *ah_out) {
     const Dual<2> tn0_sq = dconst<2>(t0 * t0) - c * (h0 * h0);
     const Dual<2> dmh = mh - dconst<2>(m0);
     const Dual<2> dmh2 = dmh * dmh;
@@ -131,17 +135,47 @@ __device__ __forceinline__ int oct_time_dual(float h, float h0, float t0, float 
     }
     const Dual<2> tn_sq = th * th - c * (h * h);
     if (th.v == 0.0f || tn0_sq.v == 0.0f) return kDivisionByZero;
-    const Dual<2> ah = ((tn_sq * t0) / (th * tn0_sq)) * slope;
-    *out = th + ah * (dconst<2>(m) - mh);
+    *th_out = th;
+    *ah_out = ((tn_sq * t0) / (th * tn0_sq)) * slope;
     return kNoError;
+}
+
+// The dual terms of one half offset for one parameter point: displaced midpoint, th and
+// ah (all independent of the trace's midpoint), or the error that stopped them.
+struct OctDualTerms {
+    Dual<2> mh, th, ah;
+    int err;
+};
+
+__device__ __forceinline__ OctDualTerms oct_dual_terms(float h, float h0, float t0, float m0,
+                                                       Dual<2> c, Dual<2> slope) {
+    OctDualTerms r;
+    r.err = oct_mh_dual(h, h0, t0, m0, c, slope, &r.mh);
+    if (r.err == kNoError) r.err = oct_time_terms_dual(h, h0, t0, m0, r.mh, c, slope, &r.th, &r.ah);
+    return r;
+}
+
+// The traveltime at midpoint m from the offset's terms.
+__device__ __forceinline__ Dual<2> oct_time_from_terms(const OctDualTerms &q, float m) {
+    return q.th + q.ah * (dconst<2>(m) - q.mh);
+}
+
+// The point's (c, slope) as duals seeded for d/dc and d/dslope.
+__device__ __forceinline__ void oct_seed_duals(const Candidate &k, Dual<2> *c, Dual<2> *sl) {
+    *c = dconst<2>(k.c);
+    *sl = dconst<2>(k.p1);
+    c->d[0] = 1.0f;
+    sl->d[1] = 1.0f;
 }
 
 // Traveltime and dt/dtheta (theta in storage form).  The value and the acceptance are
 // traveltime()'s, bit for bit, so a lane keeps the same windows as the search kernels.
+// OCT: with use_kept, `kept` holds the dual terms of the trace's half offset for this
+// point (the same bits oct_dual_terms gives for it).
 template <int MODEL, bool FAST>
 __device__ __forceinline__ bool traveltime_grad(const TraceGeom &tg, float t0, float t0sq, float m0,
                                                 float h0, float apm, const Candidate &k, float *t,
-                                                float *dtd) {
+                                                float *dtd, bool use_kept, const OctDualTerms &kept) {
     if constexpr (MODEL == HSS_CMP) {
         bool r;
         traveltime<MODEL, FAST>(tg, t0, t0sq, m0, h0, apm, k, t, &r);
@@ -162,12 +196,16 @@ __device__ __forceinline__ bool traveltime_grad(const TraceGeom &tg, float t0, f
         // kernels), derivatives from the dual-number twin of the same formulas
         bool r;
         if (!traveltime<MODEL, FAST>(tg, t0, t0sq, m0, h0, apm, k, t, &r)) return false;
-        Dual<2> c = dconst<2>(k.c), sl = dconst<2>(k.p1);
-        c.d[0] = 1.0f;
-        sl.d[1] = 1.0f;
-        Dual<2> mh, tt;
-        if (oct_mh_dual(tg.h, h0, t0, m0, c, sl, &mh) == kNoError &&
-            oct_time_dual(tg.h, h0, t0, m0, tg.m, mh, c, sl, &tt) == kNoError) {
+        OctDualTerms q;
+        if (use_kept) {
+            q = kept;
+        } else {
+            Dual<2> c, sl;
+            oct_seed_duals(k, &c, &sl);
+            q = oct_dual_terms(tg.h, h0, t0, m0, c, sl);
+        }
+        if (q.err == kNoError) {
+            const Dual<2> tt = oct_time_from_terms(q, tg.m);
             dtd[0] = tt.d[0];
             dtd[1] = tt.d[1];
         } else {
@@ -202,6 +240,15 @@ __device__ __forceinline__ void evaluate_grad(const EvalCtx &e, float t0, float 
     float den = 0.0f;
     unsigned used = 0;
     const int lane = threadIdx.x & (kWave - 1);
+    // OCT: the point's dual terms of the current half offset, kept while consecutive
+    // traces share the offset's bit pattern (a wave-uniform test) when the list comes in
+    // runs of one offset (grouped by offset); in the reference's order each trace computes
+    // its own, only when the aperture keeps it
+    bool keep = false;
+    if constexpr (MODEL == HSS_OCT) keep = oct_offset_runs_long(e);
+    OctDualTerms kept{};
+    uint32_t kept_key = 0;
+    bool kept_valid = false;
     for (int j0 = 0; j0 < e.cnt; j0 += kWave) {
         const int mine = j0 + lane;
         const bool have = mine < e.cnt;
@@ -219,7 +266,18 @@ __device__ __forceinline__ void evaluate_grad(const EvalCtx &e, float t0, float 
             const float sc =
                 scaled ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc_l), jj)) : 1.0f;
             float t, dtd[D > 0 ? D : 1];
-            if (!traveltime_grad<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k, &t, dtd)) continue;
+            if constexpr (MODEL == HSS_OCT) {
+                if (keep && (!kept_valid || __float_as_uint(tg.h) != kept_key)) {
+                    Dual<2> c, sl;
+                    oct_seed_duals(k, &c, &sl);
+                    kept = oct_dual_terms(tg.h, e.h0, t0, e.m0, c, sl);
+                    kept_key = __float_as_uint(tg.h);
+                    kept_valid = true;
+                }
+            }
+            if (!traveltime_grad<MODEL, FASTIDX>(tg, t0, t0sq, e.m0, e.h0, e.apm, k, &t, dtd, keep,
+                                                 kept))
+                continue;
             int first;
             float f;
             if (!window_index<FASTIDX>(t, e.dt, e.inv_dt, e.ib, e.tau, &first, &f)) continue;

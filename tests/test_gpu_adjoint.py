@@ -96,6 +96,58 @@ def test_gradient_matches_fp64_oracle(model, storage):
     assert bad == 0, (bad, total, fails[:4])
 
 
+@pytest.mark.parametrize("storage", [F32, F16])
+def test_oct_gradient_over_offset_grouped_lists(storage):
+    """OCT lists grouped by half offset (how the host layer passes them to the non-exact
+    variants): the adjoint keeps each point's dual traveltime terms across a run of one
+    offset (the same bits as computing them per trace, so the gradient over a list is the
+    same whichever way the terms come).  Against the FP64 oracle over the same list, to
+    the bounds above, at the points of the line-order test (seed 3 + OCT), so the two
+    tests compare both trace loops at the same parameters.  (Other random points can fall
+    where S is ~1e-3 and the gradient cancels to a few percent in FP32, in either loop.)"""
+    import torch
+    line, tau, b, c, m0, h0, apm, lo, hi = problem(OCT)
+    lists = []
+    for g in range(len(b)):
+        idx = np.arange(b[g], b[g] + c[g])
+        idx = idx[np.argsort(line.hoff[idx], kind="stable")]
+        h = line.hoff[idx].view(np.uint32)
+        runs = 1 + np.count_nonzero(h[1:] != h[:-1])
+        assert 4 * runs <= len(idx), g  # long runs: the kernel keeps the terms
+        lists.append(idx)
+    counts = np.array([len(x) for x in lists])
+    begins = np.r_[0, np.cumsum(counts)[:-1]]
+    dl = capi.DeviceLine(line.samples, line.mid, line.hoff, line.dt, tau, storage=storage)
+    mp = capi.DeviceMidpoints(begins, counts, m0, index=np.concatenate(lists), h0=h0, apm=apm)
+    pts = random_points(OCT, len(b), line.ns, lo, hi, seed=3 + OCT)
+    out = capi.semblance_gradient(dl, mp, OCT, torch.from_numpy(pts).cuda())
+    torch.cuda.synchronize()
+    out = out.cpu().numpy()
+    rng = np.random.default_rng(5)
+    bad = total = 0
+    fails = []
+    for g, idx in enumerate(lists):
+        og = O.Gather(dl.decoded[idx], line.mid[idx], line.hoff[idx], line.dt, tau,
+                      m0=float(m0[g]), h0=h0, apm=apm)
+        samples = rng.choice(np.arange(20, line.ns - 20), size=60, replace=False)
+        ref = [O.eval_grad_f64(O.OCT, og, int(s), pts[g][2:, s].astype(np.float64),
+                               f32_index=True) for s in samples]
+        S64 = np.array([r[0] for r in ref])
+        G64 = np.array([r[2] for r in ref])
+        S32 = out[g][0, samples]
+        G32 = out[g][2:, samples].T
+        scale = np.median(np.abs(G64), axis=0) + 1e-30
+        for i in range(len(samples)):
+            total += 1
+            okS = abs(S32[i] - S64[i]) <= 1e-5 * abs(S64[i]) + 1e-6
+            okG = np.all(np.abs(G32[i] - G64[i]) <= 1e-3 * np.abs(G64[i]) + 1e-3 * scale)
+            if not (okS and okG):
+                bad += 1
+                fails.append((g, int(samples[i]), float(S32[i]), S64[i], G32[i].tolist(),
+                              G64[i].tolist()))
+    assert total > 0 and bad == 0, (bad, total, fails[:4])
+
+
 @pytest.mark.parametrize("model", [CMP, ZOCRS, OCT])
 def test_polish_never_decreases_and_is_consistent(model):
     """From a coarse grid result: every polished S >= the S at the start point (same
