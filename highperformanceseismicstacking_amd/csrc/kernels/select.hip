@@ -35,9 +35,74 @@ struct SelectArgs {
     unsigned char *mask;
 };
 
-__global__ __launch_bounds__(256) void select_traces_kernel(SelectArgs a) {
+constexpr int kSelectThreads = 256;
+
+// The first test of an OCT trace: a midpoint distance beyond apm + sqrt(h^2 + h0^2)
+// (plus the rounding margin) cannot pass for any candidate.
+__device__ __forceinline__ bool oct_within_reach(float m, float h, float m0, float h0, float apm) {
+    const float reach = sqrtf(h * h + h0 * h0);
+    const float margin = 1e-4f * (fabsf(m0) + fabsf(m) + reach + apm) + 1e-3f;
+    return fabsf(m - m0) <= apm + reach + margin;
+}
+
+// DE start populations: the candidates of a midpoint are the same for every trace, and a
+// workgroup serves one midpoint, so it draws each (t0 sample, individual) candidate once
+// -- kSelectThreads at a time into LDS -- and every thread whose trace is still open
+// tests them.  The mask is the OR over all candidates, so neither the order nor the early
+// exit changes it; the workgroup stops when no trace is open.  Every thread of the
+// workgroup takes part in the barriers (those past the last trace have nothing open).
+__device__ void select_oct_de_start(const SelectArgs &a, int i, int g) {
+    __shared__ float cand_c[kSelectThreads], cand_s[kSelectThreads];
+    const bool live = i < a.ntr;
+    const float m0 = a.m0[g];
+    const float m = live ? a.mid[i] : 0.0f;
+    const float h = live ? a.hoff[i] : 0.0f;
+    bool open = live && oct_within_reach(m, h, m0, a.h0, a.apm);
+    unsigned char used = 0;
+    const int NP = a.oct.NP;
+    const long long total = (long long)a.ns * NP;
+    for (long long q0 = 0; q0 < total; q0 += kSelectThreads) {
+        if (!__syncthreads_or(open)) break;  // (also the barrier before cand_* is rewritten)
+        const long long q = q0 + threadIdx.x;
+        if (q < total) {
+            const int s = (int)(q / NP), p = (int)(q - (long long)s * NP);
+            uint32_t r[4];
+            philox4x32_10((uint32_t)s, a.oct.stream_base + (uint32_t)g, (uint32_t)p, 0u, a.key0,
+                          a.key1, r);
+            const float v = a.oct.lower[0] + u01(r[0]) * (a.oct.upper[0] - a.oct.lower[0]);
+            cand_c[threadIdx.x] = 4.0f / (v * v);
+            cand_s[threadIdx.x] = a.oct.lower[1] + u01(r[1]) * (a.oct.upper[1] - a.oct.lower[1]);
+        }
+        __syncthreads();
+        if (open) {
+            const int n = (int)min((long long)kSelectThreads, total - q0);
+            int s = (int)(q0 / NP), p = (int)(q0 - (long long)s * NP);
+            for (int j = 0; j < n; ++j) {
+                float mh;
+                const int err =
+                    oct_displaced_midpoint(h, a.h0, (float)s * a.dt, m0, cand_c[j], cand_s[j], &mh);
+                if (err == kNoError && fabsf(m - mh) <= a.apm) {
+                    used = 1;
+                    open = false;
+                    break;
+                }
+                if (++p == NP) {
+                    p = 0;
+                    ++s;
+                }
+            }
+        }
+    }
+    if (live) a.mask[(size_t)g * a.ntr + i] = used;
+}
+
+__global__ __launch_bounds__(kSelectThreads) void select_traces_kernel(SelectArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = blockIdx.y;
+    if (a.model == HSS_OCT && a.oct.mode == HSS_OCT_DE_START) {  // (uniform over the launch)
+        select_oct_de_start(a, i, g);
+        return;
+    }
     if (i >= a.ntr) return;
     const float m0 = a.m0[g];
     const float m = a.mid[i];
@@ -48,9 +113,7 @@ __global__ __launch_bounds__(256) void select_traces_kernel(SelectArgs a) {
         used = fabsf(m0 - m) <= a.apm;
     } else {
         const float h = a.hoff[i];
-        const float reach = sqrtf(h * h + a.h0 * a.h0);
-        const float margin = 1e-4f * (fabsf(m0) + fabsf(m) + reach + a.apm) + 1e-3f;
-        if (fabsf(m - m0) <= a.apm + reach + margin) {
+        if (oct_within_reach(m, h, m0, a.h0, a.apm)) {
             const int npairs = a.oct.mode == HSS_OCT_PER_SAMPLE ? 1
                              : (a.oct.mode == HSS_OCT_DE_START ? a.oct.NP : a.oct.npairs);
             for (int s = 0; !used && s < a.ns; ++s) {
@@ -66,13 +129,7 @@ __global__ __launch_bounds__(256) void select_traces_kernel(SelectArgs a) {
                         c = 4.0f / (v * v);
                         slope = vp[a.ns + s];
                     } else {
-                        // the start population of differential evolution (de_impl.hpp)
-                        uint32_t r[4];
-                        philox4x32_10((uint32_t)s, a.oct.stream_base + (uint32_t)g, (uint32_t)p, 0u,
-                                      a.key0, a.key1, r);
-                        const float v = a.oct.lower[0] + u01(r[0]) * (a.oct.upper[0] - a.oct.lower[0]);
-                        c = 4.0f / (v * v);
-                        slope = a.oct.lower[1] + u01(r[1]) * (a.oct.upper[1] - a.oct.lower[1]);
+                        continue;  // (HSS_OCT_DE_START takes select_oct_de_start)
                     }
                     float mh;
                     const int err = oct_displaced_midpoint(h, a.h0, t0, m0, c, slope, &mh);
@@ -123,7 +180,7 @@ extern "C" int hss_select_traces(const hss_traces *traces, int model, int32_t n,
         }
     }
     if (n == 0 || traces->ntr == 0) return HSS_OK;
-    dim3 grid((traces->ntr + 255) / 256, n);
-    hipLaunchKernelGGL(select_traces_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+    dim3 grid((traces->ntr + kSelectThreads - 1) / kSelectThreads, n);
+    hipLaunchKernelGGL(select_traces_kernel, grid, dim3(kSelectThreads), 0, (hipStream_t)stream, a);
     return hip_status(hipGetLastError());
 }
