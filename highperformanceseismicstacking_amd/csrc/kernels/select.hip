@@ -95,6 +95,7 @@ __device__ void select_oct(const SelectArgs &a, int i, int g) {
     __shared__ uint32_t hbits[kSelectThreads];
     __shared__ float off_h[kSelectOffsets];
     __shared__ int slot_of[kSelectThreads];
+    __shared__ int row_open[kSelectOffsets];
     __shared__ int n_off;
     const int tid = (int)threadIdx.x;
     const bool live = i < a.ntr;
@@ -133,6 +134,13 @@ __device__ void select_oct(const SelectArgs &a, int i, int g) {
     const long long total = (long long)a.ns * NP;
     for (long long q0 = 0; q0 < total; q0 += kSelectThreads) {
         if (!__syncthreads_or(open)) break;  // (also the barrier before LDS is rewritten)
+        if (tabled) {
+            // only the offsets some open trace still has get their row computed
+            if (tid < kSelectOffsets) row_open[tid] = 0;
+            __syncthreads();
+            if (open) row_open[my_row] = 1;
+            __syncthreads();
+        }
         const long long q = q0 + tid;
         if (q < total) {
             const int s = (int)(q / NP), p = (int)(q - (long long)s * NP);
@@ -141,6 +149,7 @@ __device__ void select_oct(const SelectArgs &a, int i, int g) {
             if (tabled) {
                 const float t0 = (float)s * a.dt;
                 for (int row = 0; row < n_off; ++row) {
+                    if (!row_open[row]) continue;
                     float mh;
                     const int err = oct_displaced_midpoint(off_h[row], a.h0, t0, m0, c, slope, &mh);
                     mh_tab[row][tid] = err == kNoError ? mh : __int_as_float(0x7fc00000);
