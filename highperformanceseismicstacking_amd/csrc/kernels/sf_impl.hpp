@@ -14,6 +14,7 @@
 #pragma once
 
 #include "eval.hpp"
+#include "eval_compact.hpp"
 #include "grid_impl.hpp"
 #include "launch_util.hpp"
 
@@ -69,6 +70,7 @@ __attribute__((amdgpu_waves_per_eu(sf_min_waves<MODEL, W, VARIANT>(), 8))) void 
     const float *vp = a.vparams + (size_t)g * NP * ns;
 
     __shared__ float seg_lds[kGridWaves][kSegFloats];
+    __shared__ OctTraceRec trace_recs[kGridWaves][kWave];
     float best = -1.0f, bstack = 0.0f;
     int bidx = -1;
     unsigned long long rejected = 0;
@@ -104,7 +106,23 @@ __attribute__((amdgpu_waves_per_eu(sf_min_waves<MODEL, W, VARIANT>(), 8))) void 
         };
         using T = std::true_type;
         using F = std::false_type;
-        if (MODEL == HSS_OCT && keep) {
+        bool walked = false;
+        if constexpr (MODEL == HSS_OCT && VARIANT != HSS_KAHAN) {
+            if (keep) {
+                // each lane walks only the traces its own aperture keeps (the compacted
+                // walk of differential evolution; the same sums in the same order)
+                const Candidate kc[1] = {k};
+                const bool on[1] = {true};
+                float s1[1], t1[1];
+                evaluate_oct_compact<W, VARIANT, STORAGE, true, 1>(
+                    e, t0n, kc, on, (float)n * e.dt, valid, trace_recs[wave], s1, t1, &rej);
+                S = s1[0];
+                stk = t1[0];
+                walked = true;
+            }
+        }
+        if (walked) {
+        } else if (MODEL == HSS_OCT && keep) {
             if (unscaled16)
                 eval(T{}, F{});
             else
