@@ -91,7 +91,7 @@ typedef struct hss_traces {
  * sums: HSS_EXACT equals the reference bit for bit when the list is in line order.  The
  * OCT kernels (grid, DE, stretch-free, adjoint) keep the offset-dependent half of the
  * traveltime while consecutive traces share a half offset (the same bits as computing it
- * again), so lists grouped by half offset run 1.8-2.3x faster; the non-exact variants of
+ * again), so lists grouped by half offset run 1.8-3x faster; the non-exact variants of
  * the host classes pass them so, which changes the sums' rounding only.
  *
  * `order` (no counterpart in the reference; NULL = 0, 1, ..., n-1) is the order in which
